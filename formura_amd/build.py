@@ -1,0 +1,109 @@
+"""Build the HIP libraries of the bundled programs (and of user programs).
+
+``python -m formura_amd.build`` generates ``<name>.h`` / ``<name>.hip`` for every
+entry of ``PROGRAMS`` into ``formura_amd/_build/<name>/`` and compiles
+``lib<name>.so`` for gfx950 with hipcc.  Built libraries are loaded in place
+(they are not installed anywhere else).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import hashlib
+import os
+import subprocess
+import sys
+from typing import Dict, List, Optional
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+PROG_DIR = os.path.join(PKG, "programs")
+BUILD_DIR = os.path.join(PKG, "_build")
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("FORMURA_HIP_ARCH", "gfx950")
+
+# -ffp-contract=off: no FMA contraction, so every + - * / is rounded as in the
+# reference's C.  The packed-fp32 feature is switched off for all HIP code.
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-variable", "-Wno-unused-but-set-variable", "-Wno-unused-parameter"]
+
+# name -> (program, config, schedule)
+PROGRAMS: Dict[str, tuple] = {
+    "heat3d_16": ("heat3d.fmr", "heat3d_16.yaml", "auto"),
+    "heat3d_16_tb": ("heat3d.fmr", "heat3d_16_tb.yaml", "auto"),
+    "heat3d_32": ("heat3d.fmr", "heat3d_32.yaml", "auto"),
+    "heat3d_64": ("heat3d.fmr", "heat3d_64.yaml", "auto"),
+    "heat3d_64_tb": ("heat3d.fmr", "heat3d_64_tb.yaml", "auto"),
+    "heat3d_256": ("heat3d.fmr", "heat3d_256.yaml", "auto"),
+    "heat3d_512": ("heat3d.fmr", "heat3d_512.yaml", "auto"),
+    "gs_64": ("grayscott2d.fmr", "gs_64.yaml", "auto"),
+    "gs_2048": ("grayscott2d.fmr", "gs_2048.yaml", "auto"),
+    "gs_8192": ("grayscott2d.fmr", "gs_8192.yaml", "auto"),
+}
+
+
+def lib_path(name: str) -> str:
+    return os.path.join(BUILD_DIR, name, "lib%s.so" % name)
+
+
+def header_path(name: str) -> str:
+    return os.path.join(BUILD_DIR, name, "%s.h" % name)
+
+
+def generate_program(name: str, fmr: str, yaml: str, schedule: str = "auto", out_dir: Optional[str] = None):
+    from .emit_hip import generate
+    h, src, mm = generate(fmr, yaml, name, schedule)
+    d = out_dir or os.path.join(BUILD_DIR, name)
+    os.makedirs(d, exist_ok=True)
+    for fn, text in (("%s.h" % name, h), ("%s.hip" % name, src)):
+        p = os.path.join(d, fn)
+        if not os.path.exists(p) or open(p).read() != text:
+            with open(p, "w") as f:
+                f.write(text)
+    return d
+
+
+def compile_library(name: str, src_dir: str, force: bool = False) -> str:
+    src = os.path.join(src_dir, "%s.hip" % name)
+    out = os.path.join(src_dir, "lib%s.so" % name)
+    deps = [src, os.path.join(src_dir, "%s.h" % name), os.path.join(CSRC, "formura_hip_rt.h"),
+            os.path.join(INCLUDE, "formura_hip.h")]
+    stamp = hashlib.sha1(("\0".join(HIP_FLAGS) + "".join(open(p).read() for p in deps)).encode()).hexdigest()
+    stamp_file = out + ".stamp"
+    if not force and os.path.exists(out) and os.path.exists(stamp_file) and open(stamp_file).read() == stamp:
+        return out
+    cmd = [HIPCC] + HIP_FLAGS + ["-I", CSRC, "-I", INCLUDE, "-I", src_dir, "-o", out + ".tmp", src]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (name, " ".join(cmd), r.stderr[-6000:]))
+    os.replace(out + ".tmp", out)
+    with open(stamp_file, "w") as f:
+        f.write(stamp)
+    return out
+
+
+def build_one(name: str, force: bool = False) -> str:
+    fmr, yaml, schedule = PROGRAMS[name]
+    d = generate_program(name, os.path.join(PROG_DIR, fmr), os.path.join(PROG_DIR, yaml), schedule)
+    return compile_library(name, d, force=force)
+
+
+def build_all(names: Optional[List[str]] = None, force: bool = False, jobs: int = 0) -> List[str]:
+    names = names or list(PROGRAMS)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    outs = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(build_one, n, force): n for n in names}
+        for f in cf.as_completed(futs):
+            outs.append(f.result())
+    return outs
+
+
+if __name__ == "__main__":
+    args = [a for a in sys.argv[1:] if not a.startswith("-")]
+    force = "--force" in sys.argv
+    for p in build_all(args or None, force=force):
+        print(p)

@@ -1,0 +1,345 @@
+"""Code emission: the drop-in C header and the HIP library of one program.
+
+The header reproduces the layout the reference emits (``Encode.hs:11-27``,
+``Templates.hs:23-116``): the ``#define``s, ``Formura_Grid_Struct`` (fields in
+``Data.Map`` key order, Templates.hs:53-73), ``Formura_Navi`` (field order of
+``mkNavi``, Templates.hs:81-115), ``extern Formura_Grid_Struct formura_data``
+and the prototypes of ``to_pos_*``, ``Formura_Init``, ``Formura_Forward`` and
+``Formura_Finalize``. A user ``main.c`` written against the reference compiles
+and links against the HIP library unchanged.
+
+The library replaces the generated OpenMP/MPI C:
+
+* ``Formura_Setup`` (the init graph) runs on the host, as a loop nest built the
+  way ``mkKernel`` builds it (Templates.hs:426-487), and the state is uploaded;
+* each ``Formura_Forward`` advances ``temporal_blocking_interval`` (or one) steps
+  on the GPU.  The halo of the reference (``buff[j] = data[(j-2s) mod N]``,
+  Templates.hs:336-346 / Functions.hs:265-290) becomes periodic wrap-around
+  addressing in HBM, and the output keeps the reference's drifting frame:
+  output index ``i`` of a step holds position ``i + off(range) - 2s`` of the
+  input, and ``offset_a`` decreases by ``s`` per step (Templates.hs:328,346).
+* a manifest temporary ``formura_mnX`` (Templates.hs:442-457) is evaluated over
+  the whole periodic domain and indexed by input position, which gives every
+  consumer (``LoadCursor``, :474) the same value the reference computes.
+
+The per-point arithmetic is the micro-instruction DAG of the manifest node in
+micro-id order, with the C types of Templates.hs:459-467, and no reassociation:
+HIP code is compiled with ``-ffp-contract=off``.  Sub-expressions whose inputs
+are all literals are folded on the host in IEEE double (the reference's C
+evaluates them at run time with the same correctly rounded operations; ``pow``
+and ``exp`` use the host libm, as the reference's C does).
+"""
+from __future__ import annotations
+
+import json
+import math
+import struct
+from fractions import Fraction
+from typing import Callable, Dict, List, Optional
+
+from . import syntax as S
+from .manifest import MMNode, MMProgram, MMRange, calc_range
+
+
+class EmitError(Exception):
+    pass
+
+
+CTYPE = {"Rational": "double", "double": "double", "float": "float", "int": "int",
+         "bool": "bool", "void": "void", "real": "double", "Real": "double"}
+
+
+def c_elem_type(t: S.Type) -> str:
+    """``mapType`` of the state struct (Templates.hs:63-70)."""
+    if isinstance(t, S.GridType):
+        return c_elem_type(t.elem)
+    if isinstance(t, S.ElemType):
+        if t.name == "Rational":
+            return "double"
+        if t.name in ("int", "float", "double"):
+            return t.name
+        return t.name
+    raise EmitError("Unable to translate type to C: %r" % (t,))
+
+
+def show_double(x: float) -> str:
+    """A C literal that denotes exactly the double ``x``."""
+    if math.isnan(x):
+        return "(0.0/0.0)"
+    if math.isinf(x):
+        return "(1.0/0.0)" if x > 0 else "(-1.0/0.0)"
+    r = repr(float(x))
+    if "e" not in r and "." not in r and "n" not in r:
+        r += ".0"
+    return r
+
+
+def _f32(x: float) -> float:
+    return struct.unpack("f", struct.pack("f", x))[0] if math.isfinite(x) else x
+
+
+# --- constant folding in IEEE double -------------------------------------------
+
+def _div(a, b):
+    try:
+        return a / b
+    except ZeroDivisionError:
+        if a != a or a == 0:
+            return float("nan")
+        return math.copysign(float("inf"), a) * math.copysign(1.0, b)
+
+
+def _pow(a, b):
+    try:
+        return math.pow(a, b)
+    except (OverflowError, ValueError):
+        return float("nan") if a < 0 else float("inf")
+
+
+_EXTERN_FOLD = {"sqrt": math.sqrt, "exp": math.exp, "fabs": math.fabs, "sin": math.sin,
+                "cos": math.cos, "tan": math.tan, "log": math.log, "atan": math.atan,
+                "floor": math.floor, "ceil": math.ceil, "tanh": math.tanh}
+
+
+def _fold(m, vals, typ):
+    """Fold one micro instruction whose operands are constants; None if not foldable."""
+    k = m.inst[0]
+    try:
+        if k == "Imm":
+            v = float(Fraction(m.inst[1]))
+        elif k == "Uniop":
+            a = vals[m.inst[2]]
+            op = m.inst[1]
+            if op == "-":
+                v = -a
+            elif op == "+":
+                v = a
+            elif op.startswith("external-call/"):
+                f = _EXTERN_FOLD.get(op.split("/", 1)[1])
+                if f is None:
+                    return None
+                v = float(f(float(a)))
+            else:
+                return None
+        elif k == "Binop":
+            a, b = vals[m.inst[2]], vals[m.inst[3]]
+            op = m.inst[1]
+            if isinstance(a, bool) or isinstance(b, bool):
+                if op not in ("&&", "||", "==", "!="):
+                    a, b = float(a), float(b)
+            if op == "+":
+                v = a + b
+            elif op == "-":
+                v = a - b
+            elif op == "*":
+                v = a * b
+            elif op == "/":
+                v = _div(a, b)
+            elif op == "**":
+                v = _pow(float(a), float(b))
+            elif op == "<":
+                v = a < b
+            elif op == "<=":
+                v = a <= b
+            elif op == ">":
+                v = a > b
+            elif op == ">=":
+                v = a >= b
+            elif op == "==":
+                v = a == b
+            elif op == "!=":
+                v = a != b
+            elif op == "&&":
+                v = bool(a) and bool(b)
+            elif op == "||":
+                v = bool(a) or bool(b)
+            else:
+                return None
+        elif k == "Triop":
+            c, a, b = vals[m.inst[2]], vals[m.inst[3]], vals[m.inst[4]]
+            v = a if c else b
+        else:
+            return None
+    except (ValueError, OverflowError):
+        return None
+    if typ == "bool":
+        return bool(v)
+    if typ == "float":
+        return _f32(float(v))
+    if typ in ("double", "Rational", "real", "Real"):
+        return float(v)
+    return None
+
+
+def _const_literal(v, typ):
+    if typ == "bool":
+        return "true" if v else "false"
+    if typ == "float":
+        return show_double(v) + "f" if math.isfinite(v) else "((float)%s)" % show_double(v)
+    return show_double(v)
+
+
+class Body:
+    """C statements for one manifest node's micro DAG."""
+
+    def __init__(self, node: MMNode, load: Callable, tmp_load: Callable, index: Callable,
+                 store: Callable, prefix: str, fold: bool = True, const_kw="const "):
+        self.lines: List[str] = []
+        vals: Dict[int, object] = {}
+        for j, m in enumerate(node.micro):
+            typ = m.typ
+            ct = CTYPE.get(typ, typ)
+            name = "%s%d" % (prefix, j)
+            k = m.inst[0]
+            if k == "Store":
+                self.lines.append(store(m.inst[1], "%s%d" % (prefix, m.inst[2])))
+                continue
+            if fold:
+                ops = _operands(m)
+                if all(o in vals for o in ops):
+                    v = _fold(m, vals, typ)
+                    if v is not None:
+                        vals[j] = v
+                        self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, _const_literal(v, typ)))
+                        continue
+            a = lambda i: "%s%d" % (prefix, i)
+            if k == "Imm":
+                rhs = show_double(float(Fraction(m.inst[1])))
+            elif k == "LoadCursorStatic":
+                rhs = load(m.inst[2], m.inst[1])
+            elif k == "LoadCursor":
+                rhs = tmp_load(m.inst[2], m.inst[1])
+            elif k == "LoadIndex":
+                rhs = index(m.inst[1])
+            elif k == "Uniop":
+                op = m.inst[1]
+                if op.startswith("external-call/"):
+                    rhs = "%s((double)%s)" % (op.split("/", 1)[1], a(m.inst[2]))
+                else:
+                    rhs = "%s%s" % (op, a(m.inst[2]))
+            elif k == "Binop":
+                op = m.inst[1]
+                if op == "**":
+                    rhs = "pow((double)%s,(double)%s)" % (a(m.inst[2]), a(m.inst[3]))
+                else:
+                    rhs = "%s%s%s" % (a(m.inst[2]), op, a(m.inst[3]))
+            elif k == "Triop":
+                rhs = "%s?%s:%s" % (a(m.inst[2]), a(m.inst[3]), a(m.inst[4]))
+            else:
+                raise EmitError("Unimplemented for keyword: %s" % k)
+            self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, rhs))
+        self.result = "%s%d" % (prefix, len(node.micro) - 1)
+
+
+def _operands(m):
+    k = m.inst[0]
+    if k == "Uniop":
+        return [m.inst[2]]
+    if k == "Binop":
+        return [m.inst[2], m.inst[3]]
+    if k == "Triop":
+        return [m.inst[2], m.inst[3], m.inst[4]]
+    if k == "Imm":
+        return []
+    return [None]  # loads are never constant
+
+
+# --- the drop-in header ---------------------------------------------------------
+
+def navi_fields(om) -> List[tuple]:
+    """``mkNavi`` without MPI (Templates.hs:81-115): (name, ctype, init expression)."""
+    cfg = om.config
+    axes = om.axes
+    out = [("time_step", "int", "0")]
+    out += [("lower_" + a, "int", "0") for a in axes]
+    out += [("upper_" + a, "int", str(n)) for a, n in zip(axes, cfg.grid_per_node)]
+    out += [("space_interval_" + a, "double", show_double(d)) for a, d in zip(axes, cfg.space_interval)]
+    out += [("my_rank", "int", "0")]
+    out += [("offset_" + a, "int", "0") for a in axes]
+    out += [("length_" + a, "double", show_double(float(l))) for a, l in zip(axes, cfg.length_per_node)]
+    out += [("total_grid_" + a, "int", str(n)) for a, n in zip(axes, cfg.grid_per_node)]
+    return out
+
+
+def state_fields(om) -> List[tuple]:
+    return [(name, c_elem_type(t)) for name, t in om.state_signature.items()]
+
+
+def emit_header(om, with_ext: bool = True) -> str:
+    cfg = om.config
+    if cfg.mpi_shape is not None:
+        raise EmitError("mpi_shape is set: the HIP backend runs one rank on one GPU; omit mpi_shape "
+                        "(the periodic halo is wrapped in HBM)")
+    dims = "".join("[%d]" % n for n in cfg.grid_per_node)
+    L = ["#pragma once",
+         "#include <stdio.h>", "#include <stdlib.h>", "#include <stdbool.h>", "#include <math.h>",
+         "#define Ns %d" % cfg.sleeve]
+    L += ["#define L%d %d" % (i + 1, n) for i, n in enumerate(cfg.grid_per_node)]
+    L.append("typedef struct {")
+    L += ["%s %s%s;" % (ct, name, dims) for name, ct in state_fields(om)]
+    L.append("} %s;" % om.grid_struct_type_name)
+    L.append("typedef struct {")
+    L += ["%s %s;" % (ct, name) for name, ct, _ in navi_fields(om)]
+    L.append("} Formura_Navi;")
+    L.append("extern %s %s;" % (om.grid_struct_type_name, om.grid_struct_instance_name))
+    L += ["double to_pos_%s(int,Formura_Navi);" % a for a in om.axes]
+    L.append("void Formura_Init(int *,char ***,Formura_Navi *);")
+    L.append("void Formura_Forward(Formura_Navi *);")
+    L.append("void Formura_Finalize();")
+    if with_ext:
+        L.append("#include \"formura_hip.h\"")
+    return "\n".join(L) + "\n"
+
+
+# --- the HIP library -------------------------------------------------------------
+
+class GraphPlan:
+    """What one MM graph (step, filter or first_step) needs on the GPU."""
+
+    def __init__(self, mm: Dict[int, MMNode], sleeve: int, dim: int, tag: str):
+        self.mm = mm
+        self.sleeve = sleeve
+        self.tag = tag
+        self.rng = calc_range(mm)
+        self.temps = [k for k in sorted(mm) if not _is_void(mm[k].typ)]
+        self.stores = [k for k in sorted(mm) if _is_void(mm[k].typ)]
+        for k in self.stores:
+            if self.rng[k].size != 2 * sleeve:
+                raise EmitError("manifest node %s of %s has stencil range [%d,%d], which the reference only "
+                                "evaluates consistently when it spans 2*sleeve=%d"
+                                % (mm[k].name, tag, self.rng[k].lower, self.rng[k].upper, 2 * sleeve))
+        for k in self.temps:
+            if any(m.inst[0] == "LoadIndex" for m in mm[k].micro):
+                raise EmitError("index access inside the manifest temporary %s is not supported" % mm[k].name)
+            if not isinstance(mm[k].typ, S.GridType):
+                raise EmitError("manifest temporary %s is not a grid" % mm[k].name)
+        self.max_reach = 0
+        for k, node in mm.items():
+            base = self.rng[k].offset - 2 * sleeve if k in self.stores else 0
+            for m in node.micro:
+                if m.inst[0] in ("LoadCursorStatic", "LoadCursor"):
+                    for c in m.inst[1]:
+                        self.max_reach = max(self.max_reach, abs(c + base))
+
+    def temp_ctype(self, k):
+        return c_elem_type(self.mm[k].typ)
+
+
+def _is_void(t):
+    return isinstance(t, S.ElemType) and t.name == "void"
+
+
+def lib_description(om, name: str, schedule: str) -> dict:
+    cfg = om.config
+    return {
+        "name": name,
+        "dimension": om.dimension,
+        "axes": om.axes,
+        "grid": cfg.grid_per_node,
+        "sleeve": cfg.sleeve,
+        "steps_per_forward": cfg.steps_per_forward,
+        "fields": [{"name": n, "ctype": t} for n, t in state_fields(om)],
+        "navi": [{"name": n, "ctype": t} for n, t, _ in navi_fields(om)],
+        "filter_interval": cfg.filter_interval,
+        "schedule": schedule,
+    }

@@ -1,0 +1,375 @@
+"""HIP library emitter: kernels, host orchestration and the C ABI of one program.
+
+The generated ``.hip`` file is self-contained apart from the runtime header
+``csrc/formura_hip_rt.h``.  It defines ``formura_data`` and the reference API
+(``Formura_Init``/``Formura_Forward``/``Formura_Finalize``/``to_pos_*``), and the
+device-resident extension declared in ``include/formura_hip.h``.
+
+Kernel schedules (chosen per program, see ``choose_schedule``):
+
+``flat``     one work-item per output point, every load through the cache
+             hierarchy with periodic wrap; manifest temporaries get their own
+             kernels (any dimension, any program).
+``stream3d`` 3-D programs without temporaries: a workgroup owns a
+             (TY x TZ) column tile and streams along the slowest axis; input
+             planes are staged in an LDS ring of ``2s+1`` planes per field with
+             their halos, so each input value is read from HBM once per tile.
+``tile2d``   2-D programs without temporaries: a workgroup stages a tile plus
+             a halo of ``s*k`` in LDS and advances ``k`` steps there (temporal
+             blocking), writing only the ``k``-step result.
+"""
+from __future__ import annotations
+
+import json
+from typing import Dict, List
+
+from . import syntax as S
+from .emit import (Body, EmitError, GraphPlan, c_elem_type, emit_header,
+                   lib_description, navi_fields, show_double, state_fields, CTYPE)
+from .manifest import MMNode, calc_range
+
+
+def _idx_expr(dim, names):
+    # row-major linear index with compile-time extents N1..Nd
+    e = names[0]
+    for k in range(1, dim):
+        e = "(%s)*N%d+%s" % (e, k + 1, names[k])
+    return e
+
+
+class LibEmitter:
+    def __init__(self, mm, name: str, schedule: str = "auto", options=None):
+        self.mm = mm
+        self.om = mm.om
+        self.name = name
+        self.dim = self.om.dimension
+        self.cfg = self.om.config
+        self.N = self.cfg.grid_per_node
+        self.fields = state_fields(self.om)
+        self.field_index = {n: i for i, (n, _) in enumerate(self.fields)}
+        self.options = dict(options or {})
+        if self.cfg.mpi_shape is not None:
+            raise EmitError("mpi_shape is set: the HIP backend runs one rank on one GPU; omit mpi_shape")
+        self.step = GraphPlan(mm.step, self.cfg.sleeve, self.dim, "step")
+        self.filter = GraphPlan(mm.filter, self.cfg.filter_sleeve, self.dim, "filter") if mm.filter else None
+        self.first = GraphPlan(mm.first_step, self.cfg.sleeve0, self.dim, "first_step") if mm.first_step else None
+        if self.cfg.blocking is not None:
+            for plan in (self.step,):
+                for k in plan.stores:
+                    if any(m.inst[0] == "LoadIndex" for m in plan.mm[k].micro):
+                        raise EmitError("index access in a temporally blocked step is not supported")
+        self.schedule = choose_schedule(self, schedule)
+
+    # -- helpers -----------------------------------------------------------------
+    def wrap_fn(self, k, reach):
+        n = self.N[k]
+        if reach < n:
+            return "fmr_wrap<%d>" % n
+        return "fmr_mod<%d>" % n
+
+    def axes_vars(self, base="i"):
+        return ["%s%d" % (base, k + 1) for k in range(self.dim)]
+
+    # -- flat schedule --------------------------------------------------------------
+    def flat_kernels(self, plan: GraphPlan) -> List[str]:
+        out = []
+        dim = self.dim
+        ivs = self.axes_vars()
+        wrap = [self.wrap_fn(k, plan.max_reach + 2 * plan.sleeve) for k in range(dim)]
+        tmp_params = "".join(", %s* __restrict__ t_%d" % (plan.temp_ctype(k), k) for k in plan.temps)
+        tmp_cparams = "".join(", const %s* __restrict__ t_%d" % (plan.temp_ctype(k), k) for k in plan.temps)
+        navi_params = "".join(", int off_%d" % (k + 1) for k in range(dim))
+
+        def decompose():
+            lines = ["const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;",
+                     "if (t >= NTOT) return;"]
+            rem = "t"
+            for k in range(dim - 1, -1, -1):
+                if k == 0:
+                    lines.append("const int %s = (int)(%s);" % (ivs[k], rem))
+                else:
+                    lines.append("const int %s = (int)(%s %% N%d);" % (ivs[k], rem, k + 1))
+                    rem = "(%s / N%d)" % (rem, k + 1)
+            return lines
+
+        def loaders(base):
+            def pos(c):
+                return [("%s(%s%+d)" % (wrap[k], ivs[k], c[k] + base[k])) if (c[k] + base[k]) else ivs[k]
+                        for k in range(dim)]
+
+            def load(field, c):
+                fi = self.field_index[field]
+                return "FLD%d(st_in)[%s]" % (fi, _idx_expr(dim, pos(c)))
+
+            def tmp_load(omid, c):
+                return "t_%d[%s]" % (omid, _idx_expr(dim, pos(c)))
+
+            return load, tmp_load
+
+        for k in plan.temps:
+            node = plan.mm[k]
+            load, tmp_load = loaders([0] * dim)
+            body = Body(node, load, tmp_load, lambda ax: "0", lambda n, v: "", "a")
+            body.lines.append("t_%d[t] = %s;" % (k, body.result))
+            deps = [d for d in plan.temps if d < k]
+            params = "".join(", const %s* __restrict__ t_%d" % (plan.temp_ctype(d), d) for d in deps)
+            out.append(
+                "__global__ void __launch_bounds__(256) k_%s_tmp%d(const char* __restrict__ st_in, %s* __restrict__ t_%d%s) {\n"
+                % (plan.tag, k, plan.temp_ctype(k), k, params)
+                + "\n".join("  " + l for l in decompose() + body.lines) + "\n}\n")
+        # one fused kernel for every Store node
+        lines = decompose()
+        for k in plan.stores:
+            node = plan.mm[k]
+            r = plan.rng[k]
+            base = [r.offset - 2 * plan.sleeve] * dim
+            load, tmp_load = loaders(base)
+
+            def store(fname, v):
+                fi = self.field_index[fname]
+                return "FLD%d(st_out)[t] = %s;" % (fi, v)
+
+            def index(ax):
+                return "(%s+off_%d)" % (ivs[ax], ax + 1)
+
+            body = Body(node, load, tmp_load, index, store, "s%d_" % k)
+            lines.append("{")
+            lines += ["  " + l for l in body.lines]
+            lines.append("}")
+        out.append(
+            "__global__ void __launch_bounds__(256) k_%s_store(const char* __restrict__ st_in, char* __restrict__ st_out%s%s) {\n"
+            % (plan.tag, tmp_cparams, navi_params)
+            + "\n".join("  " + l for l in lines) + "\n}\n")
+        return out
+
+    def flat_launch(self, plan: GraphPlan) -> str:
+        """Host function that runs one application of ``plan`` (one step)."""
+        dim = self.dim
+        L = ["static void run_%s(const char* in, char* out, const Formura_Navi& n, hipStream_t s) {" % plan.tag,
+             "  const unsigned nb = (unsigned)((NTOT + 255) / 256);"]
+        for k in plan.temps:
+            deps = [d for d in plan.temps if d < k]
+            L.append("  hipLaunchKernelGGL(k_%s_tmp%d, dim3(nb), dim3(256), 0, s, in, (%s*)g_tmp_%s[%d]%s);"
+                     % (plan.tag, k, plan.temp_ctype(k), plan.tag, plan.temps.index(k),
+                        "".join(", (const %s*)g_tmp_%s[%d]" % (plan.temp_ctype(d), plan.tag, plan.temps.index(d)) for d in deps)))
+        args = "".join(", (const %s*)g_tmp_%s[%d]" % (plan.temp_ctype(d), plan.tag, plan.temps.index(d)) for d in plan.temps)
+        offs = "".join(", n.offset_%s" % a for a in self.om.axes)
+        L.append("  hipLaunchKernelGGL(k_%s_store, dim3(nb), dim3(256), 0, s, in, out%s%s);" % (plan.tag, args, offs))
+        L.append("  FMR_CHECK(hipGetLastError());")
+        L.append("}")
+        return "\n".join(L)
+
+    # -- host Setup (init graph), mirroring mkKernel with sleeve 0 ------------------------
+    def setup_function(self) -> str:
+        dim = self.dim
+        mm = self.mm.init
+        rng = calc_range(mm)
+        ivs = self.axes_vars()
+        L = ["static void Formura_Setup(Formura_Navi n) {"]
+        inst = self.om.grid_struct_instance_name
+        for k in sorted(mm):
+            node = mm[k]
+            r = rng[k]
+            ext = [n - r.size for n in self.N]
+            is_tmp = not _is_void(node.typ)
+            if is_tmp:
+                ct = c_elem_type(node.typ)
+                L.append("  static %s formura_mn%d%s;" % (ct, k, "".join("[%d]" % max(e, 1) for e in ext)))
+
+            def load(field, c, r=r):
+                return "%s.%s%s" % (inst, field, "".join("[%s%+d]" % (ivs[a], c[a] + r.offset) for a in range(dim)))
+
+            def tmp_load(omid, c, r=r):
+                ro = rng[omid].offset
+                return "formura_mn%d%s" % (omid, "".join("[%s%+d]" % (ivs[a], c[a] + r.offset - ro) for a in range(dim)))
+
+            def index(ax):
+                return "(double)(%s+n.offset_%s+0)" % (ivs[ax], self.om.axes[ax])
+
+            def store(fname, v, k=k):
+                if is_tmp:
+                    return "formura_mn%d%s = %s;" % (k, "".join("[%s]" % i for i in ivs), v)
+                return "%s.%s%s = %s;" % (inst, fname, "".join("[%s]" % i for i in ivs), v)
+
+            body = Body(node, load, tmp_load, index, store, "a", fold=False)
+            lines = list(body.lines)
+            if is_tmp:
+                lines.append(store(None, body.result))
+            loop = "  " + "".join("for (int %s = 0; %s < %d; %s++) " % (ivs[a], ivs[a], ext[a], ivs[a]) for a in range(dim))
+            L.append(loop + "{")
+            L += ["    " + l for l in lines]
+            L.append("  }")
+        L.append("}")
+        return "\n".join(L)
+
+    # -- whole file ---------------------------------------------------------------
+    def emit(self) -> str:
+        om, dim = self.om, self.dim
+        tname = om.grid_struct_type_name
+        inst = om.grid_struct_instance_name
+        desc = lib_description(om, self.name, self.schedule.name)
+        L = ["// Generated by formura_amd from %s. Do not edit." % self.name,
+             "#include \"formura_hip_rt.h\"",
+             "extern \"C\" {",
+             "#include \"%s.h\"" % self.name,
+             "}",
+             "extern \"C\" { %s %s; }" % (tname, inst),
+             "namespace {"]
+        L += ["constexpr int N%d = %d;" % (k + 1, n) for k, n in enumerate(self.N)]
+        L.append("constexpr long long NTOT = %s;" % "*".join("(long long)%d" % n for n in self.N))
+        L.append("constexpr int SLEEVE = %d;" % self.cfg.sleeve)
+        L.append("constexpr int STEPS_PER_FORWARD = %d;" % self.cfg.steps_per_forward)
+        L.append("constexpr size_t STATE_BYTES = sizeof(%s);" % tname)
+        for i, (n, ct) in enumerate(self.fields):
+            L.append("#define FLD%d(st) ((%s%s*)((st) + offsetof(%s, %s)))"
+                     % (i, "const " if False else "", ct, tname, n))
+        L.append("const char* const kDescription = R\"FMRJSON(%s)FMRJSON\";" % json.dumps(desc))
+        plans = [p for p in (self.step, self.filter, self.first) if p is not None]
+        for p in plans:
+            L.append("void* g_tmp_%s[%d] = {0};" % (p.tag, max(1, len(p.temps))))
+            L.append("const size_t g_tmp_bytes_%s[%d] = {%s};" % (
+                p.tag, max(1, len(p.temps)),
+                ", ".join("sizeof(%s) * (size_t)NTOT" % p.temp_ctype(k) for k in p.temps) or "0"))
+        for p in plans:
+            if p is self.step:
+                L += self.schedule.kernels(self, p)
+            else:
+                L += self.flat_kernels(p)
+        for p in plans:
+            if p is self.step:
+                L.append(self.schedule.launch(self, p))
+            else:
+                L.append(self.flat_launch(p))
+        L.append(self.setup_function())
+        L.append("}  // namespace")
+        L.append(self.api(plans))
+        return "\n".join(L) + "\n"
+
+    def api(self, plans) -> str:
+        om = self.om
+        inst = om.grid_struct_instance_name
+        L = []
+        for a in om.axes:
+            L.append("extern \"C\" double to_pos_%s(int i, Formura_Navi n) {\n"
+                     "  return n.space_interval_%s*((i+n.offset_%s)%%n.total_grid_%s);\n}" % (a, a, a, a))
+        shift = lambda s: "\n".join(
+            "  n->offset_%s = (n->offset_%s - %d + n->total_grid_%s)%%n->total_grid_%s;" % (a, a, s, a, a)
+            for a in om.axes)
+        tmp_alloc = "\n".join(
+            "  for (int k = 0; k < %d; ++k) g_tmp_%s[k] = fmr_rt::device_alloc(g_tmp_bytes_%s[k]);"
+            % (len(p.temps), p.tag, p.tag) for p in plans if p.temps)
+        tmp_free = "\n".join(
+            "  for (int k = 0; k < %d; ++k) { fmr_rt::device_free(g_tmp_%s[k]); g_tmp_%s[k] = 0; }"
+            % (len(p.temps), p.tag, p.tag) for p in plans if p.temps)
+        first = ""
+        if self.first:
+            first = ("  run_first_step(g_rt.cur, g_rt.nxt, *n, g_rt.stream); g_rt.swap();\n" + shift(self.cfg.sleeve0)
+                     + "\n  if (!g_rt.resident) fmr_rt::download(g_rt, &%s);" % inst)
+        filt = ""
+        if self.filter:
+            filt = ("    if (n->time_step %% %d == 0) {\n      run_filter(g_rt.cur, g_rt.nxt, *n, g_rt.stream); g_rt.swap();\n%s\n    }"
+                    % (self.cfg.filter_interval, "\n".join("  " + l for l in shift(self.cfg.filter_sleeve).split("\n"))))
+        navi_init = "\n".join("  n->%s = %s;" % (f, v) for f, _, v in navi_fields(om))
+        sched_multi = self.schedule.multi_step_call(self)
+        L.append("""
+static fmr_rt::Runtime g_rt;
+
+static void fmr_forward_device(Formura_Navi *n) {
+%(multi)s
+%(shift)s
+%(filt)s
+  n->time_step += STEPS_PER_FORWARD;
+}
+
+extern "C" void Formura_Init(int *argc, char ***argv, Formura_Navi *n) {
+  (void)argc; (void)argv;
+%(navi_init)s
+  Formura_Setup(*n);
+  fmr_rt::init(g_rt, STATE_BYTES);
+%(tmp_alloc)s
+  fmr_rt::upload(g_rt, &%(inst)s);
+%(first)s
+}
+
+extern "C" void Formura_Forward(Formura_Navi *n) {
+  if (!g_rt.ready) fmr_rt::fatal("Formura_Forward called before Formura_Init");
+  if (!g_rt.resident) fmr_rt::upload(g_rt, &%(inst)s);
+  fmr_forward_device(n);
+  if (!g_rt.resident) fmr_rt::download(g_rt, &%(inst)s);
+}
+
+extern "C" void Formura_Forward_Steps(Formura_Navi *n, int k) {
+  if (!g_rt.ready) fmr_rt::fatal("Formura_Forward_Steps called before Formura_Init");
+  if (k < 0) fmr_rt::fatal("Formura_Forward_Steps: negative count");
+  if (!g_rt.resident) fmr_rt::upload(g_rt, &%(inst)s);
+  fmr_rt::timer_start(g_rt);
+  for (int i = 0; i < k; ++i) fmr_forward_device(n);
+  fmr_rt::timer_stop(g_rt);
+  if (!g_rt.resident) fmr_rt::download(g_rt, &%(inst)s);
+}
+
+extern "C" void Formura_Finalize() {
+  if (!g_rt.ready) return;
+  if (g_rt.resident) fmr_rt::download(g_rt, &%(inst)s);
+%(tmp_free)s
+  fmr_rt::shutdown(g_rt);
+}
+
+extern "C" void Formura_HIP_Set_Resident(int on) { g_rt.resident = on ? 1 : 0; }
+extern "C" void Formura_HIP_Upload(void) { if (g_rt.ready) fmr_rt::upload(g_rt, &%(inst)s); }
+extern "C" void Formura_HIP_Download(void) { if (g_rt.ready) fmr_rt::download(g_rt, &%(inst)s); }
+extern "C" const char *Formura_HIP_Describe(void) { return kDescription; }
+extern "C" void *Formura_HIP_State_Device(void) { return g_rt.cur; }
+extern "C" void *Formura_HIP_Stream(void) { return (void*)g_rt.stream; }
+extern "C" double Formura_HIP_Last_Elapsed_Ms(void) { return g_rt.last_ms; }
+extern "C" long long Formura_HIP_Kernel_Launches_Per_Forward(void) { return %(launches)s; }
+""" % dict(multi=sched_multi, shift=shift(self.cfg.sleeve * self.cfg.steps_per_forward), filt=filt,
+           navi_init=navi_init, tmp_alloc=tmp_alloc, tmp_free=tmp_free, first=first, inst=inst,
+           launches=self.schedule.launches_per_forward(self)))
+        return "\n".join(L)
+
+
+def _is_void(t):
+    return isinstance(t, S.ElemType) and t.name == "void"
+
+
+# --- schedules ---------------------------------------------------------------------
+
+class FlatSchedule:
+    name = "flat"
+
+    def kernels(self, em: LibEmitter, plan):
+        return em.flat_kernels(plan)
+
+    def launch(self, em: LibEmitter, plan):
+        return em.flat_launch(plan)
+
+    def multi_step_call(self, em: LibEmitter):
+        return ("  for (int it = 0; it < STEPS_PER_FORWARD; ++it) {\n"
+                "    run_step(g_rt.cur, g_rt.nxt, *n, g_rt.stream); g_rt.swap();\n  }")
+
+    def launches_per_forward(self, em: LibEmitter):
+        return "STEPS_PER_FORWARD * %d" % (len(em.step.temps) + 1)
+
+
+def choose_schedule(em: LibEmitter, want: str):
+    if want in ("auto", "flat"):
+        return FlatSchedule()
+    raise EmitError("unknown schedule %r" % want)
+
+
+# --- driver --------------------------------------------------------------------------
+
+def generate(fmr_path: str, yaml_path: str, name: str, schedule: str = "auto"):
+    """Front-end + emitters: returns (header text, hip source text, mm program)."""
+    from .config import read_config
+    from .parser import parse_program
+    from .translate import gen_om_program
+    from .manifest import gen_mm_program
+    nc = read_config(yaml_path)
+    with open(fmr_path) as f:
+        prog = parse_program(f.read(), nc, fmr_path)
+    om = gen_om_program(prog)
+    mm = gen_mm_program(om)
+    em = LibEmitter(mm, name, schedule)
+    return emit_header(om), em.emit(), mm

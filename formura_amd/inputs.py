@@ -1,0 +1,51 @@
+"""Seeded synthetic inputs for the bundled programs.
+
+All random fields come from SplitMix64 (Steele, Lea and Flood, "Fast splittable
+pseudorandom number generators", OOPSLA 2014): state += 0x9E3779B97F4A7C15,
+then the xor-shift-multiply finaliser; a uniform double in [0,1) is the top 53
+bits times 2^-53.  Values are drawn in C row-major order of the field, so the
+same seed gives the same field on every platform.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int, start: int = 0) -> np.ndarray:
+    """The ``n`` outputs ``start .. start+n-1`` of SplitMix64 seeded with ``seed``."""
+    with np.errstate(over="ignore"):
+        k = np.arange(start + 1, start + n + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + k * _GAMMA
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, shape, start: int = 0) -> np.ndarray:
+    """U[0,1) doubles in row-major order."""
+    n = int(np.prod(shape))
+    z = splitmix64(seed, n, start)
+    return ((z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)).reshape(shape)
+
+
+def heat_field(seed: int, shape) -> dict:
+    """Heat equation input: q i.i.d. U[0,1)."""
+    return {"q": uniform(seed, shape)}
+
+
+def grayscott_perturb(u: np.ndarray, v: np.ndarray, seed: int) -> dict:
+    """Seeded form of the reference driver's ``perturbate`` (examples/pearson2/main.c:13-22):
+    where ``u != 1`` add ``0.01*r`` to u and to v, r ~ U[0,1) drawn u-then-v per cell
+    in row-major order."""
+    mask = u != 1.0
+    r = uniform(seed, (u.size, 2)).reshape(u.shape + (2,))
+    u2 = u.copy()
+    v2 = v.copy()
+    u2[mask] += 0.01 * r[..., 0][mask]
+    v2[mask] += 0.01 * r[..., 1][mask]
+    return {"u": u2, "v": v2}

@@ -1,0 +1,171 @@
+"""Manifestation: OM graph -> one micro-instruction DAG per manifest node.
+
+Follows ``src/Formura/OrthotopeMachine/Manifestation.hs``:
+
+* each manifest node (``Store`` nodes and ``manifest ::`` names) becomes one
+  loop nest, visited in ascending OM id (:170-207);
+* delayed nodes are inlined at every cursor; ``Shift`` moves the cursor,
+  ``Load`` becomes ``LoadCursorStatic``, a reference to another manifest node
+  becomes ``LoadCursor`` (:116-152);
+* micro-node ids are allocated in post-order and shared per
+  ``(cursor, OM node)`` (:85-93, :104-113), so ascending id is a valid
+  evaluation order;
+
+and the range analysis of ``src/Formura/Generator/Templates.hs``:
+
+* ``calc_range`` (:388-424) is a scalar ``[lo, hi]`` over every cursor
+  component, merged through ``LoadCursor`` chains.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Dict, List, Tuple
+
+from . import syntax as S
+from .translate import Graph, OMProgram
+
+
+@dataclasses.dataclass
+class Micro:
+    inst: tuple      # ("Imm", r) ("LoadCursorStatic", cursor, name) ("LoadCursor", cursor, omid)
+                     # ("LoadIndex", axis) ("Uniop", op, a) ("Binop", op, a, b) ("Triop", op, a, b, c)
+                     # ("Store", name, a)
+    typ: str         # element type name (micro types are element types only)
+
+
+@dataclasses.dataclass
+class MMNode:
+    omid: int
+    micro: List[Micro]        # index == micro id, topological order
+    typ: S.Type               # OM node type (void for Store)
+    name: str
+
+
+def _micro_type(t: S.Type) -> str:
+    if isinstance(t, S.GridType):
+        return _micro_type(t.elem)
+    if isinstance(t, S.ElemType):
+        return t.name
+    raise ValueError("Top type encountered while manifestation")
+
+
+def _add(c, v):
+    return tuple(a + b for a, b in zip(c, v))
+
+
+def manifest_graph(g: Graph, dim: int) -> Dict[int, MMNode]:
+    """``manifestG`` (Manifestation.hs:170-207)."""
+    manifest_ids = [i for i, n in enumerate(g.nodes) if n.manifest]
+    mset = set(manifest_ids)
+    out = {}
+    for root in manifest_ids:
+        idmap: Dict[Tuple[tuple, int], int] = {}
+        micro: Dict[int, Micro] = {}
+
+        def insert(c, i, inst):
+            key = (c, i)
+            if key not in idmap:
+                idmap[key] = len(idmap)
+            j = idmap[key]
+            micro[j] = Micro(inst, _micro_type(g.nodes[i].typ))
+            return j
+
+        def code_at(c, nid):
+            if nid in mset:
+                return insert(c, nid, ("LoadCursor", c, nid))
+            return delayed_at(c, nid)
+
+        def delayed_at(c, nid):
+            inst = g.nodes[nid].inst
+            k = inst[0]
+            if k == "Imm":
+                return insert(c, nid, ("Imm", inst[1]))
+            if k == "Uniop":
+                ja = code_at(c, inst[2])
+                return insert(c, nid, ("Uniop", inst[1], ja))
+            if k == "Binop":
+                ja = code_at(c, inst[2])
+                jb = code_at(c, inst[3])
+                return insert(c, nid, ("Binop", inst[1], ja, jb))
+            if k == "Triop":
+                ja = code_at(c, inst[2])
+                jb = code_at(c, inst[3])
+                jc = code_at(c, inst[4])
+                return insert(c, nid, ("Triop", inst[1], ja, jb, jc))
+            if k == "Shift":
+                return code_at(_add(c, inst[1]), inst[2])
+            if k == "Load":
+                return insert(c, nid, ("LoadCursorStatic", c, inst[1]))
+            if k == "LoadIndex":
+                return insert(c, nid, ("LoadIndex", inst[1]))
+            if k == "Store":
+                ja = code_at(c, inst[2])
+                return insert(c, nid, ("Store", inst[1], ja))
+            raise ValueError("manifestation path unimplemented for keyword: %s" % k)
+
+        delayed_at((0,) * dim, root)
+        node = g.nodes[root]
+        out[root] = MMNode(root, [micro[j] for j in range(len(micro))], node.typ,
+                           node.name or ("mn%d" % root))
+    return out
+
+
+@dataclasses.dataclass(frozen=True)
+class MMRange:
+    lower: int = 0
+    upper: int = 0
+
+    def union(self, o: "MMRange") -> "MMRange":
+        return MMRange(min(self.lower, o.lower), max(self.upper, o.upper))
+
+    def merge(self, o: "MMRange") -> "MMRange":
+        return MMRange(self.lower + o.lower, self.upper + o.upper)
+
+    @property
+    def size(self) -> int:
+        """``toSize`` (Templates.hs:411-412)."""
+        return abs(self.lower) + abs(self.upper)
+
+    @property
+    def offset(self) -> int:
+        """``toOffset`` (Templates.hs:414-415)."""
+        return abs(self.lower)
+
+
+def _from_cursor(c) -> MMRange:
+    return MMRange(min(c), max(c))
+
+
+def calc_range(mm: Dict[int, MMNode]) -> Dict[int, MMRange]:
+    """``calcRange`` (Templates.hs:417-424)."""
+    tbl: Dict[int, MMRange] = {}
+    for omid in sorted(mm):
+        rng = MMRange()
+        for m in mm[omid].micro:
+            if m.inst[0] == "LoadCursorStatic":
+                rng = rng.union(_from_cursor(m.inst[1]))
+            elif m.inst[0] == "LoadCursor":
+                rng = rng.union(_from_cursor(m.inst[1]).merge(tbl[m.inst[2]]))
+        tbl[omid] = rng
+    return tbl
+
+
+@dataclasses.dataclass
+class MMProgram:
+    om: OMProgram
+    init: Dict[int, MMNode]
+    first_step: Dict[int, MMNode] = None
+    filter: Dict[int, MMNode] = None
+    step: Dict[int, MMNode] = None
+
+
+def gen_mm_program(om: OMProgram) -> MMProgram:
+    """``genMMProgram`` (Manifestation.hs:209-258)."""
+    d = om.dimension
+    return MMProgram(
+        om=om,
+        init=manifest_graph(om.init_graph, d),
+        first_step=manifest_graph(om.first_step_graph, d) if om.first_step_graph else None,
+        filter=manifest_graph(om.filter_graph, d) if om.filter_graph else None,
+        step=manifest_graph(om.step_graph, d),
+    )

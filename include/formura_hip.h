@@ -1,0 +1,50 @@
+/*
+ * formura_hip.h -- device-resident extension of the generated Formura API.
+ *
+ * Every library that formura_amd generates for a program exports, besides the
+ * reference API of the generated header (see the per-program headers in this
+ * directory), the entry points below.  They are declared after the program's
+ * Formura_Navi typedef (the generated header includes this file last).
+ *
+ * The reference API they sit beside (emitted by src/Formura/Generator/Templates.hs):
+ *   double to_pos_<axis>(int, Formura_Navi);        Templates.hs:221-225
+ *   void   Formura_Init(int *, char ***, Formura_Navi *);   Templates.hs:227-250
+ *   void   Formura_Forward(Formura_Navi *);         Templates.hs:295-334
+ *   void   Formura_Finalize();                      Templates.hs:383-386
+ *   extern Formura_Grid_Struct formura_data;        Templates.hs:53-61
+ *
+ * The reference is CPU-only: it has no device and no counterpart of these
+ * functions.  Its state is always host-visible, which this library reproduces
+ * by default (Formura_Forward uploads formura_data before the step and
+ * downloads it after).  Callers that keep the state in HBM switch that off.
+ * Errors follow the reference convention: a message on stderr, then exit(1).
+ */
+#pragma once
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 1: the state stays in HBM between calls; Formura_Forward no longer copies
+ * formura_data.  0 (default): every Formura_Forward syncs both ways. */
+void Formura_HIP_Set_Resident(int on);
+/* Copy host formura_data to the device state / the device state to host. */
+void Formura_HIP_Upload(void);
+void Formura_HIP_Download(void);
+/* Exactly k calls of Formura_Forward, with no host synchronisation in
+ * between (one upload before and one download after when not resident).
+ * The device time of the k calls is timed with HIP events on the library's
+ * stream and returned by Formura_HIP_Last_Elapsed_Ms. */
+void Formura_Forward_Steps(Formura_Navi *n, int k);
+double Formura_HIP_Last_Elapsed_Ms(void);
+/* JSON description of the program: grid, sleeve, fields, Navi layout. */
+const char *Formura_HIP_Describe(void);
+/* Device address of the current state image (the byte layout of
+ * formura_data), and the HIP stream the library launches on. */
+void *Formura_HIP_State_Device(void);
+void *Formura_HIP_Stream(void);
+/* Kernel launches issued by one Formura_Forward. */
+long long Formura_HIP_Kernel_Launches_Per_Forward(void);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,78 @@
+"""Build recipe for the oracle (test infrastructure only).
+
+Compiles the hand transcriptions of the reference's generated C (``*_ref.c``)
+into shared libraries under ``oracle/_build/``.  Each library exports exactly
+the API the reference's generated C exports (``Formura_Init``,
+``Formura_Forward``, ``Formura_Finalize``, ``to_pos_*``, ``formura_data``).
+
+Two builds per configuration:
+
+* ``strict`` (parity): ``-O2 -ffp-contract=off`` -- no FMA contraction, no
+  reassociation, so every operation is rounded as written;
+* ``fast`` (CPU baseline timing): the reference Makefiles' flags
+  ``-O3 -ffast-math -mavx2 -mfma -fopenmp`` (examples/diffusion2/Makefile:2,
+  examples/pearson2/Makefile:2); ``-march=native`` (sample/3d_hllc/Makefile:2)
+  is not used because the library is built here and run on another host.
+
+Usage: ``python oracle/build_oracle.py [name ...]``.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "_build")
+
+STRICT = ["-O2", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-shared", "-std=gnu11"]
+FAST = ["-O3", "-ffast-math", "-mavx2", "-mfma", "-fopenmp", "-fPIC", "-shared", "-std=gnu11"]
+
+# name -> (source, defines)
+CONFIGS = {
+    "heat3d_16": ("heat3d_ref.c", {"NX": 16}),
+    "heat3d_16_tb": ("heat3d_ref.c", {"NX": 16, "TB": 1, "NT": 2, "GB1": 4, "GB2": 4, "GB3": 4}),
+    "heat3d_32": ("heat3d_ref.c", {"NX": 32}),
+    "heat3d_64": ("heat3d_ref.c", {"NX": 64}),
+    "heat3d_64_tb": ("heat3d_ref.c", {"NX": 64, "TB": 1, "NT": 4, "GB1": 8, "GB2": 8, "GB3": 8}),
+    "heat3d_256": ("heat3d_ref.c", {"NX": 256}),
+    "gs_64": ("grayscott2d_ref.c", {"NX": 64}),
+    "gs_2048": ("grayscott2d_ref.c", {"NX": 2048}),
+}
+
+
+def lib_path(name: str, kind: str = "strict") -> str:
+    return os.path.join(OUT, "liboracle_%s_%s.so" % (name, kind))
+
+
+def build(name: str, kind: str = "strict", force: bool = False) -> str:
+    src, defs = CONFIGS[name]
+    flags = (STRICT if kind == "strict" else FAST) + ["-D%s=%s" % kv for kv in sorted(defs.items())]
+    srcp = os.path.join(HERE, src)
+    out = lib_path(name, kind)
+    stamp = hashlib.sha1((" ".join(flags) + open(srcp).read()).encode()).hexdigest()
+    if not force and os.path.exists(out) and os.path.exists(out + ".stamp") and open(out + ".stamp").read() == stamp:
+        return out
+    os.makedirs(OUT, exist_ok=True)
+    cmd = ["gcc"] + flags + ["-o", out + ".tmp", srcp, "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(out + ".tmp", out)
+    with open(out + ".stamp", "w") as f:
+        f.write(stamp)
+    return out
+
+
+def build_all(names=None, kinds=("strict", "fast")):
+    out = []
+    for n in names or CONFIGS:
+        for k in kinds:
+            out.append(build(n, k))
+    return out
+
+
+if __name__ == "__main__":
+    for p in build_all(sys.argv[1:] or None):
+        print(p)
