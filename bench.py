@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the generated Formura_Forward on one MI355X.
+
+Workload (one "step" = one Formura_Forward = one time step of the whole grid):
+2-D Gray-Scott reaction-diffusion, 2048 x 2048 periodic grid, FP64, fields u and
+v, dx = 0.01, dt = 1, Du = 2e-5, Dv = 1e-5, F = 0.04, k = 0.06; initial state
+from the program's init graph (u = 1, v = 0, central square of 205 cells with
+u = 0.5, v = 0.25) plus seeded 1 % noise on the square; 10000 steps by default.
+
+The state is resident in HBM for the timed region (Formura_HIP_Set_Resident).
+``value`` is GLUPS = cells x steps / s / 1e9.  ``roofline`` is measured on the
+dominant kernel: algorithmic bytes per launch (32 B per cell update: read u, v,
+write u', v' in FP64) over its mean launch duration from HIP events recorded on
+the library's stream around the timed launches.  ``cpu_baseline`` times the
+oracle (the transcription of the reference's generated C, compiled with the
+reference Makefile flags and OpenMP) on the same workload for a bounded number
+of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _traffic_from_profile(kernel_tag: str):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    p = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        m = re.match(r"\s*traffic_bytes_per_launch\[%s\]\s*=\s*([0-9.eE+]+)" % re.escape(kernel_tag), line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def run_gpu(name: str, steps: int, warmup: int, bytes_per_lup: float, seed: int = 7):
+    import numpy as np
+    from formura_amd import inputs
+    from formura_amd.runtime import open_program
+    lib = open_program(name)
+    lib.init()
+    if "u" in lib.fields:
+        lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), seed))
+    else:
+        lib.set_state(inputs.heat_field(seed, lib.grid))
+    lib.set_resident(True)
+    lib.upload()
+    spf = lib.layout["steps_per_forward"]
+    if warmup:
+        lib.forward_steps(max(1, warmup // spf))
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    t0 = time.perf_counter()
+    ms = lib.forward_steps(max(1, steps // spf))
+    wall = time.perf_counter() - t0
+    nsteps = max(1, steps // spf) * spf
+    cells = int(np.prod(lib.grid))
+    launches = lib.launches_per_forward() * max(1, steps // spf)
+    lib.download()
+    finite = all(bool(np.isfinite(lib.field(f)).all()) for f in lib.fields)
+    lib.finalize()
+    glups = cells * nsteps / (ms * 1e-3) / 1e9
+    return {
+        "glups": glups, "ms": ms, "wall_s": wall, "steps": nsteps, "cells": cells,
+        "launches": launches, "mean_launch_us": ms * 1e3 / launches,
+        "bytes_per_launch": bytes_per_lup * cells * nsteps / launches,
+        "schedule": lib.layout.get("schedule"), "finite": finite,
+    }
+
+
+def cpu_baseline(name: str, target_s: float = 10.0):
+    """Oracle (fast build) on the host cores, timed on a bounded number of steps."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import build_oracle
+    from formura_amd import inputs
+    from formura_amd.runtime import FormuraLib
+    from formura_amd.emit_hip import generate
+    from formura_amd.emit import lib_description
+    from formura_amd import build as B
+    fmr, yml, _ = B.PROGRAMS[name]
+    _, _, mm = generate(os.path.join(B.PROG_DIR, fmr), os.path.join(B.PROG_DIR, yml), name)
+    path = build_oracle.lib_path(name, "fast")
+    if not os.path.exists(path):
+        build_oracle.build(name, "fast")
+    lib = FormuraLib(path, lib_description(mm.om, name, ""))
+    lib.init()
+    lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), 7))
+    lib.forward(1)  # warm-up
+    cells = int(np.prod(lib.grid))
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        lib.forward(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= target_s or n >= 100000:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": cells * n / el / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": "%d steps of the %s workload (%.1f s), oracle C built with -O3 -ffast-math "
+                      "-mavx2 -mfma -fopenmp, OMP_NUM_THREADS=%d" % (n, name, el, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device-to-device copy of 256 MiB, 20 times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(20):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": a.numel(), "launches": 20}))
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the periodic halo is wrapped in HBM; "
+              "no inter-GPU exchange is built). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    name = "gs_2048"
+    r = run_gpu(name, args.steps, args.warmup, 32.0)
+    achieved = r["bytes_per_launch"] / (r["mean_launch_us"] * 1e-6) / 1e9
+    traffic = _traffic_from_profile(name)
+    line = {
+        "metric": "GLUPS & % HBM roofline, FP32/batched vs plain FP64; adjoint/forward ratio",
+        "value": round(r["glups"], 3),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": r["ms"] / r["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "grayscott2d 2048x2048 periodic, FP64 u,v, dx=0.01 dt=1 Du=2e-5 Dv=1e-5 "
+                               "F=0.04 k=0.06, seeded init, plain (bit-exact) path",
+                   "grid": [2048, 2048], "fields": 2, "schedule": r["schedule"],
+                   "bytes_per_lup": 32, "resident": True},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+    }
+    if not r["finite"]:
+        line["error"] = "non-finite state after the run"
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        try:
+            line["cpu_baseline"] = cpu_baseline(name, args.cpu_seconds)
+        except Exception as e:  # the GPU number stands without it
+            line["cpu_baseline"] = {"error": str(e)[:300]}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
