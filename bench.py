@@ -69,7 +69,7 @@ def run_gpu(name: str, steps: int, warmup: int, bytes_per_lup: float, seed: int 
     wall = time.perf_counter() - t0
     nsteps = max(1, steps // spf) * spf
     cells = int(np.prod(lib.grid))
-    launches = lib.launches_per_forward() * max(1, steps // spf)
+    launches = lib.last_launches()
     lib.download()
     finite = all(bool(np.isfinite(lib.field(f)).all()) for f in lib.fields)
     lib.finalize()

@@ -51,6 +51,7 @@ struct Runtime {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
+  long long launches = 0;  // kernel launches in the last Formura_Forward_Steps
   void swap() {
     char *t = cur;
     cur = nxt;

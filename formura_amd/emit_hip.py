@@ -145,7 +145,7 @@ class LibEmitter:
     def flat_launch(self, plan: GraphPlan) -> str:
         """Host function that runs one application of ``plan`` (one step)."""
         dim = self.dim
-        L = ["static void run_%s(const char* in, char* out, const Formura_Navi& n, hipStream_t s) {" % plan.tag,
+        L = ["static void run_%s(const char* in, char* out, const int* offs, hipStream_t s) {" % plan.tag,
              "  const unsigned nb = (unsigned)((NTOT + 255) / 256);"]
         for k in plan.temps:
             deps = [d for d in plan.temps if d < k]
@@ -153,7 +153,7 @@ class LibEmitter:
                      % (plan.tag, k, plan.temp_ctype(k), plan.tag, plan.temps.index(k),
                         "".join(", (const %s*)g_tmp_%s[%d]" % (plan.temp_ctype(d), plan.tag, plan.temps.index(d)) for d in deps)))
         args = "".join(", (const %s*)g_tmp_%s[%d]" % (plan.temp_ctype(d), plan.tag, plan.temps.index(d)) for d in plan.temps)
-        offs = "".join(", n.offset_%s" % a for a in self.om.axes)
+        offs = "".join(", offs[%d]" % k for k in range(dim))
         L.append("  hipLaunchKernelGGL(k_%s_store, dim3(nb), dim3(256), 0, s, in, out%s%s);" % (plan.tag, args, offs))
         L.append("  FMR_CHECK(hipGetLastError());")
         L.append("}")
@@ -245,6 +245,22 @@ class LibEmitter:
         L.append(self.api(plans))
         return "\n".join(L) + "\n"
 
+    def forward_body(self, shift, adv_filter) -> str:
+        axes = self.om.axes
+        if self.filter is None:
+            lines = ["  const long long steps = forwards * STEPS_PER_FORWARD;",
+                     "  fmr_advance(steps, *n);"]
+            for a in axes:
+                lines.append("  { const long long sh = ((long long)SLEEVE * steps) %% n->total_grid_%s;\n"
+                             "    n->offset_%s = (int)((n->offset_%s - sh + n->total_grid_%s) %% n->total_grid_%s); }"
+                             % (a, a, a, a, a))
+            lines.append("  n->time_step += (int)steps;")
+            return "\n".join(lines)
+        return ("  for (long long f = 0; f < forwards; ++f) {\n"
+                "    fmr_advance(STEPS_PER_FORWARD, *n);\n%s\n%s"
+                "    n->time_step += STEPS_PER_FORWARD;\n  }"
+                % ("\n".join("  " + l for l in shift(self.cfg.sleeve * self.cfg.steps_per_forward).split("\n")), adv_filter))
+
     def api(self, plans) -> str:
         om = self.om
         inst = om.grid_struct_instance_name
@@ -263,22 +279,33 @@ class LibEmitter:
             % (len(p.temps), p.tag, p.tag) for p in plans if p.temps)
         first = ""
         if self.first:
-            first = ("  run_first_step(g_rt.cur, g_rt.nxt, *n, g_rt.stream); g_rt.swap();\n" + shift(self.cfg.sleeve0)
+            first = ("  { const int offs[] = {%s}; run_first_step(g_rt.cur, g_rt.nxt, offs, g_rt.stream); g_rt.swap(); }\n" % ", ".join("n->offset_%s" % a for a in om.axes) + shift(self.cfg.sleeve0)
                      + "\n  if (!g_rt.resident) fmr_rt::download(g_rt, &%s);" % inst)
         filt = ""
         if self.filter:
             filt = ("    if (n->time_step %% %d == 0) {\n      run_filter(g_rt.cur, g_rt.nxt, *n, g_rt.stream); g_rt.swap();\n%s\n    }"
                     % (self.cfg.filter_interval, "\n".join("  " + l for l in shift(self.cfg.filter_sleeve).split("\n"))))
         navi_init = "\n".join("  n->%s = %s;" % (f, v) for f, _, v in navi_fields(om))
-        sched_multi = self.schedule.multi_step_call(self)
+        offs_list = ", ".join("n->offset_%s" % a for a in om.axes)
+        total_list = ", ".join("n->total_grid_%s" % a for a in om.axes)
+        adv_filter = ""
+        if self.filter:
+            adv_filter = ("    if (n->time_step %% %d == 0) {\n"
+                          "      { const int offs[] = {%s}; run_filter(g_rt.cur, g_rt.nxt, offs, g_rt.stream); g_rt.swap(); ++g_rt.launches; }\n%s\n    }\n"
+                          % (self.cfg.filter_interval, ", ".join("n->offset_%s" % a for a in om.axes),
+                             "\n".join("  " + l for l in shift(self.cfg.filter_sleeve).split("\n"))))
         L.append("""
 static fmr_rt::Runtime g_rt;
 
-static void fmr_forward_device(Formura_Navi *n) {
-%(multi)s
-%(shift)s
-%(filt)s
-  n->time_step += STEPS_PER_FORWARD;
+%(advance)s
+
+// Advance by `forwards` calls of Formura_Forward on the device.  Without a
+// filter hook the steps of all calls run back to back (in multi-step launches
+// where the schedule has them); the Navi bookkeeping is the closed form of
+// repeated Templates.hs:328,346 updates.
+static void fmr_forward_device(Formura_Navi *n, long long forwards) {
+  if (forwards <= 0) return;
+%(forward_body)s
 }
 
 extern "C" void Formura_Init(int *argc, char ***argv, Formura_Navi *n) {
@@ -294,7 +321,7 @@ extern "C" void Formura_Init(int *argc, char ***argv, Formura_Navi *n) {
 extern "C" void Formura_Forward(Formura_Navi *n) {
   if (!g_rt.ready) fmr_rt::fatal("Formura_Forward called before Formura_Init");
   if (!g_rt.resident) fmr_rt::upload(g_rt, &%(inst)s);
-  fmr_forward_device(n);
+  fmr_forward_device(n, 1);
   if (!g_rt.resident) fmr_rt::download(g_rt, &%(inst)s);
 }
 
@@ -302,8 +329,9 @@ extern "C" void Formura_Forward_Steps(Formura_Navi *n, int k) {
   if (!g_rt.ready) fmr_rt::fatal("Formura_Forward_Steps called before Formura_Init");
   if (k < 0) fmr_rt::fatal("Formura_Forward_Steps: negative count");
   if (!g_rt.resident) fmr_rt::upload(g_rt, &%(inst)s);
+  g_rt.launches = 0;
   fmr_rt::timer_start(g_rt);
-  for (int i = 0; i < k; ++i) fmr_forward_device(n);
+  fmr_forward_device(n, k);
   fmr_rt::timer_stop(g_rt);
   if (!g_rt.resident) fmr_rt::download(g_rt, &%(inst)s);
 }
@@ -322,10 +350,9 @@ extern "C" const char *Formura_HIP_Describe(void) { return kDescription; }
 extern "C" void *Formura_HIP_State_Device(void) { return g_rt.cur; }
 extern "C" void *Formura_HIP_Stream(void) { return (void*)g_rt.stream; }
 extern "C" double Formura_HIP_Last_Elapsed_Ms(void) { return g_rt.last_ms; }
-extern "C" long long Formura_HIP_Kernel_Launches_Per_Forward(void) { return %(launches)s; }
-""" % dict(multi=sched_multi, shift=shift(self.cfg.sleeve * self.cfg.steps_per_forward), filt=filt,
-           navi_init=navi_init, tmp_alloc=tmp_alloc, tmp_free=tmp_free, first=first, inst=inst,
-           launches=self.schedule.launches_per_forward(self)))
+extern "C" long long Formura_HIP_Last_Launches(void) { return g_rt.launches; }
+""" % dict(advance=self.schedule.advance(self), forward_body=self.forward_body(shift, adv_filter),
+           navi_init=navi_init, tmp_alloc=tmp_alloc, tmp_free=tmp_free, first=first, inst=inst))
         return "\n".join(L)
 
 
@@ -335,25 +362,169 @@ def _is_void(t):
 
 # --- schedules ---------------------------------------------------------------------
 
+def _offsets_at(axes, base="n0", t="t"):
+    """Navi offsets after t steps of sleeve SLEEVE, as C expressions."""
+    return ["(int)((((long long)%s.offset_%s - (long long)SLEEVE * %s) %% N%d + N%d) %% N%d)" % (base, a, t, k + 1, k + 1, k + 1)
+            for k, a in enumerate(axes)]
+
+
 class FlatSchedule:
+    """One launch per step (plus one per manifest temporary)."""
+
     name = "flat"
 
-    def kernels(self, em: LibEmitter, plan):
+    def kernels(self, em, plan):
         return em.flat_kernels(plan)
 
-    def launch(self, em: LibEmitter, plan):
+    def launch(self, em, plan):
         return em.flat_launch(plan)
 
-    def multi_step_call(self, em: LibEmitter):
-        return ("  for (int it = 0; it < STEPS_PER_FORWARD; ++it) {\n"
-                "    run_step(g_rt.cur, g_rt.nxt, *n, g_rt.stream); g_rt.swap();\n  }")
+    def advance(self, em):
+        offs = _offsets_at(em.om.axes)
+        return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
+                "  for (long long t = 0; t < steps; ++t) {\n"
+                "    const int offs[] = {%s};\n"
+                "    run_step(g_rt.cur, g_rt.nxt, offs, g_rt.stream); g_rt.swap();\n"
+                "    g_rt.launches += %d;\n  }\n}" % (", ".join(offs), len(em.step.temps) + 1))
 
-    def launches_per_forward(self, em: LibEmitter):
-        return "STEPS_PER_FORWARD * %d" % (len(em.step.temps) + 1)
+
+class Tile2DSchedule:
+    """2-D, no temporaries: K steps per launch in LDS (temporal blocking).
+
+    A workgroup owns a TX x TY tile of the output.  With the drifting frame an
+    output index i depends only on input indices [i-2s, i], so K steps need the
+    tile plus a low-side halo of H = 2*K*s rows and columns.  The halo tile is
+    loaded once, level j = 1..K computes local indices [2*j*s, TX+H) x
+    [2*j*s, TY+H) from level j-1 (ping-pong LDS buffers), and level K is
+    written to HBM.  Every point is evaluated with the same instruction list as
+    the one-step kernel, so the result is identical to K single steps.
+    """
+
+    name = "tile2d"
+
+    def __init__(self, tx, ty, k):
+        self.tx, self.ty, self.k = tx, ty, k
+
+    def kernels(self, em, plan):
+        out = []
+        s = plan.sleeve
+        TX, TY = self.tx, self.ty
+        fields = em.fields
+        # point function: reads level-(j-1) LDS buffers around local offset o
+        loads_used = {}
+        body_lines = []
+        params = ", ".join("const %s* __restrict__ lds%d" % (ct, i) for i, (n, ct) in enumerate(fields))
+        outs = ", ".join("%s& o%d" % (ct, i) for i, (n, ct) in enumerate(fields))
+        for k in plan.stores:
+            node = plan.mm[k]
+            r = plan.rng[k]
+            base = r.offset - 2 * s
+
+            def load(field, c, base=base):
+                fi = em.field_index[field]
+                dr, dc = c[0] + base, c[1] + base
+                return "lds%d[o + (%d)*RY + (%d)]" % (fi, dr, dc)
+
+            def store(fname, v):
+                return "o%d = %s;" % (em.field_index[fname], v)
+
+            b = Body(node, load, lambda *a: None, lambda ax: None, store, "s%d_" % k)
+            body_lines.append("{")
+            body_lines += ["  " + l for l in b.lines]
+            body_lines.append("}")
+        out.append("template <int RY>\n__device__ __forceinline__ void pt_step(%s, const int o, %s) {\n%s\n}\n"
+                   % (params, outs, "\n".join("  " + l for l in body_lines)))
+        wrap1 = em.wrap_fn(0, 2 * self.k * s + 1)
+        wrap2 = em.wrap_fn(1, 2 * self.k * s + 1)
+        lds = "\n".join("  __shared__ %s B%d[2][RX * RY];" % (ct, i) for i, (n, ct) in enumerate(fields))
+        load_tile = "\n".join("    B%d[0][idx] = FLD%d(st_in)[g];" % (i, i) for i in range(len(fields)))
+        srcs = ", ".join("B%d[src]" % i for i in range(len(fields)))
+        decl_o = " ".join("%s o%d;" % (ct, i) for i, (n, ct) in enumerate(fields))
+        refs_o = ", ".join("o%d" % i for i in range(len(fields)))
+        st_lds = " ".join("B%d[dst][o] = o%d;" % (i, i) for i in range(len(fields)))
+        st_glb = " ".join("FLD%d(st_out)[g] = o%d;" % (i, i) for i in range(len(fields)))
+        out.append("""
+template <int K>
+__global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
+  constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
+  constexpr int NTX = N1 / TX, NTY = N2 / TY, NT = NTX * NTY;
+%(lds)s
+  // XCD-aware tile order: workgroup ids are dealt round-robin to the 8 XCDs, so
+  // give each XCD a contiguous band of tiles (shared halos stay in its L2).
+  const int b = blockIdx.x;
+  const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
+  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+  for (int idx = threadIdx.x; idx < RX * RY; idx += 256) {
+    const int r = idx / RY, c = idx - r * RY;
+    const long long g = (long long)%(wrap1)s(x0 - H + r) * N2 + %(wrap2)s(y0 - H + c);
+%(load_tile)s
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 1; j <= K; ++j) {
+    const int src = (j - 1) & 1, dst = j & 1;
+    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
+    for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
+      const int rr = idx / W;
+      const int r = lo + rr, c = lo + (idx - rr * W);
+      const int o = r * RY + c;
+      %(decl_o)s
+      pt_step<RY>(%(srcs)s, o, %(refs_o)s);
+      if (j < K) {
+        %(st_lds)s
+      } else {
+        const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+        %(st_glb)s
+      }
+    }
+    if (j < K) __syncthreads();
+  }
+}
+""" % dict(TX=TX, TY=TY, lds=lds, wrap1=wrap1, wrap2=wrap2, load_tile=load_tile, decl_o=decl_o,
+           srcs=srcs, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb))
+        return out
+
+    def launch(self, em, plan):
+        nt = (em.N[0] // self.tx) * (em.N[1] // self.ty)
+        return ("static void run_step_tile(int k, const char* in, char* out, hipStream_t s) {\n"
+                "  switch (k) {\n%s  default: fmr_rt::fatal(\"bad step count\");\n  }\n"
+                "  FMR_CHECK(hipGetLastError());\n}\n"
+                % "".join("  case %d: hipLaunchKernelGGL(k_step_tile<%d>, dim3(%d), dim3(256), 0, s, in, out); break;\n"
+                          % (k, k, nt) for k in range(1, self.k + 1)))
+
+    def advance(self, em):
+        return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
+                "  (void)n0;\n"
+                "  while (steps > 0) {\n"
+                "    const int k = steps >= %d ? %d : (int)steps;\n"
+                "    run_step_tile(k, g_rt.cur, g_rt.nxt, g_rt.stream); g_rt.swap();\n"
+                "    g_rt.launches += 1; steps -= k;\n  }\n}" % (self.k, self.k))
 
 
-def choose_schedule(em: LibEmitter, want: str):
-    if want in ("auto", "flat"):
+def _tile2d_ok(em, tx, ty, k):
+    plan = em.step
+    if em.dim != 2 or plan.temps:
+        return False
+    if any(m.inst[0] == "LoadIndex" for kk in plan.stores for m in plan.mm[kk].micro):
+        return False
+    if em.N[0] % tx or em.N[1] % ty:
+        return False
+    lds = sum(8 if ct == "double" else 4 for _, ct in em.fields) * 2 * (tx + 2 * k * plan.sleeve) * (ty + 2 * k * plan.sleeve)
+    return lds <= 64 * 1024
+
+
+def choose_schedule(em, want: str):
+    opts = em.cfg.extra
+    if want == "flat":
+        return FlatSchedule()
+    tile = opts.get("hip_tile", [32, 32])
+    k = int(opts.get("hip_steps_per_launch", 4))
+    if want in ("auto", "tile2d"):
+        for tx, ty, kk in ((int(tile[0]), int(tile[1]), k), (32, 32, k), (16, 16, k), (32, 32, 1), (16, 16, 1), (8, 8, 1)):
+            if _tile2d_ok(em, tx, ty, kk):
+                return Tile2DSchedule(tx, ty, kk)
+        if want == "tile2d":
+            raise EmitError("the tile2d schedule does not apply to this program")
         return FlatSchedule()
     raise EmitError("unknown schedule %r" % want)
 

@@ -95,7 +95,7 @@ class FormuraLib:
             L.Formura_HIP_Last_Elapsed_Ms.restype = ctypes.c_double
             L.Formura_HIP_State_Device.restype = ctypes.c_void_p
             L.Formura_HIP_Stream.restype = ctypes.c_void_p
-            L.Formura_HIP_Kernel_Launches_Per_Forward.restype = ctypes.c_longlong
+            L.Formura_HIP_Last_Launches.restype = ctypes.c_longlong
         self.navi = Navi()
 
     # -- the reference API ------------------------------------------------------
@@ -152,9 +152,10 @@ class FormuraLib:
         self.lib.Formura_Forward_Steps(ctypes.byref(self.navi), int(k))
         return self.lib.Formura_HIP_Last_Elapsed_Ms()
 
-    def launches_per_forward(self) -> int:
+    def last_launches(self) -> int:
+        """Kernel launches issued by the last ``forward_steps`` call."""
         self._need_hip()
-        return int(self.lib.Formura_HIP_Kernel_Launches_Per_Forward())
+        return int(self.lib.Formura_HIP_Last_Launches())
 
 
 def open_program(name: str) -> FormuraLib:

@@ -42,8 +42,8 @@ const char *Formura_HIP_Describe(void);
  * formura_data), and the HIP stream the library launches on. */
 void *Formura_HIP_State_Device(void);
 void *Formura_HIP_Stream(void);
-/* Kernel launches issued by one Formura_Forward. */
-long long Formura_HIP_Kernel_Launches_Per_Forward(void);
+/* Kernel launches issued by the last Formura_Forward_Steps call. */
+long long Formura_HIP_Last_Launches(void);
 
 #ifdef __cplusplus
 }

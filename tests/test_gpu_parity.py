@@ -84,6 +84,8 @@ def test_grayscott_bit_exact(name, steps):
     _assert_same(o, h)
 
 
-def test_resident_steps_equal_forward_calls():
-    o, h = _run_pair("gs_64", "gs_64", _gs_seed, 50, resident=True)
+@pytest.mark.parametrize("name,steps", [("gs_64", 50), ("gs_2048", 43)])
+def test_resident_multi_step_launches(name, steps):
+    """Formura_Forward_Steps runs several steps per launch (temporal blocking in LDS)."""
+    o, h = _run_pair(name, name, _gs_seed, steps, resident=True)
     _assert_same(o, h)
