@@ -1,0 +1,131 @@
+"""Autotuner for the kernel schedule of a program (tile shape, steps per launch).
+
+Two phases, so that compilation stays off the GPU box:
+
+``python -m formura_amd.autotune build NAME...``
+    generates and compiles one library per candidate schedule
+    (``emit_hip.candidate_specs``) under ``_build/NAME/variants/``;
+``python -m formura_amd.autotune run NAME... --out DIR``
+    on the GPU: runs every variant on the same seeded state for the same
+    number of time steps (state resident in HBM), checks that every variant's
+    final state is bit-identical to the first one's, and writes
+    ``DIR/NAME.json`` with the GLUPS of every variant and the fastest spec.
+
+Copying that JSON to ``formura_amd/tuning/NAME.json`` makes the build use the
+winner (``emit_hip.choose_schedule``).
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+import concurrent.futures as cf
+
+from . import build as B
+
+
+def _tag(spec: dict) -> str:
+    return "_".join(str(spec[k]) for k in sorted(spec)) if spec else "default"
+
+
+def variant_dir(name: str) -> str:
+    return os.path.join(B.BUILD_DIR, name, "variants")
+
+
+def build_variants(name: str, jobs: int = 8):
+    from .emit_hip import LibEmitter, candidate_specs, generate
+    from .emit import emit_header
+    fmr, yml, _ = B.PROGRAMS[name]
+    _, _, mm = generate(os.path.join(B.PROG_DIR, fmr), os.path.join(B.PROG_DIR, yml), name)
+    base = LibEmitter(mm, name, "flat")
+    specs = candidate_specs(base)
+    todo = []
+    for spec in specs:
+        d = os.path.join(variant_dir(name), _tag(spec))
+        os.makedirs(d, exist_ok=True)
+        src = LibEmitter(mm, name, spec).emit()
+        for fn, text in (("%s.h" % name, emit_header(mm.om)), ("%s.hip" % name, src)):
+            with open(os.path.join(d, fn), "w") as f:
+                f.write(text)
+        with open(os.path.join(d, "spec.json"), "w") as f:
+            json.dump(spec, f)
+        todo.append(d)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda d: B.compile_library(name, d), todo))
+    return todo
+
+
+def run_variants(name: str, out_dir: str, seconds: float = 0.3, seed: int = 5):
+    import numpy as np
+    from . import inputs
+    from .runtime import FormuraLib
+    root = variant_dir(name)
+    dirs = sorted(d for d in os.listdir(root) if os.path.exists(os.path.join(root, d, "lib%s.so" % name)))
+    results, ref_state, ref_tag = [], None, None
+    steps = None
+    for d in dirs:
+        spec = json.load(open(os.path.join(root, d, "spec.json")))
+        lib = FormuraLib(os.path.join(root, d, "lib%s.so" % name))
+        lib.init()
+        if "u" in lib.fields and "v" in lib.fields:
+            lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), seed))
+        else:
+            lib.set_state({f: inputs.uniform(seed + i, lib.grid) for i, f in enumerate(lib.fields)})
+        lib.set_resident(True)
+        lib.upload()
+        cells = int(np.prod(lib.grid))
+        if steps is None:  # size the run from the first variant: ~seconds of work
+            ms = lib.forward_steps(4)
+            per = max(ms / 4, 1e-3)
+            steps = int(max(24, min(20000, seconds * 1e3 / per))) // 24 * 24
+            lib.download()
+            lib.finalize()
+            lib.init()
+            if "u" in lib.fields and "v" in lib.fields:
+                lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), seed))
+            else:
+                lib.set_state({f: inputs.uniform(seed + i, lib.grid) for i, f in enumerate(lib.fields)})
+            lib.upload()
+        lib.forward_steps(min(steps, 48))  # warm-up
+        ms = lib.forward_steps(steps)
+        lib.download()
+        st = lib.state()
+        if ref_state is None:
+            ref_state, ref_tag = st, d
+            same = True
+        else:
+            same = all(np.array_equal(st[f], ref_state[f]) for f in st)
+        glups = cells * (steps + min(steps, 48)) / 1e9 / 1 if False else cells * steps / (ms * 1e-3) / 1e9
+        results.append({"spec": spec, "tag": d, "glups": glups, "ms": ms, "steps": steps,
+                        "launches": lib.last_launches(), "identical_to_first": bool(same)})
+        print("%-40s %8.2f GLUPS  %s" % (d, glups, "ok" if same else "MISMATCH"), flush=True)
+        lib.finalize()
+    good = [r for r in results if r["identical_to_first"]]
+    best = max(good, key=lambda r: r["glups"]) if good else None
+    out = {"name": name, "best": best["spec"] if best else None, "results": results,
+           "reference_variant": ref_tag, "steps_per_variant": steps}
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, "%s.json" % name), "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("phase", choices=["build", "run"])
+    ap.add_argument("names", nargs="+")
+    ap.add_argument("--out", default="gpurun_out/tuning")
+    ap.add_argument("--seconds", type=float, default=0.3)
+    a = ap.parse_args()
+    for n in a.names:
+        if a.phase == "build":
+            print(n, len(build_variants(n)), "variants", flush=True)
+        else:
+            run_variants(n, a.out, a.seconds)
+
+
+if __name__ == "__main__":
+    main()

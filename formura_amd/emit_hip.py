@@ -501,6 +501,142 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
                 "    g_rt.launches += 1; steps -= k;\n  }\n}" % (self.k, self.k))
 
 
+class Stream3DSchedule:
+    """3-D, no temporaries: stream along the slowest axis with K levels in LDS.
+
+    A workgroup owns a TY x TZ column tile and a chunk of CX planes along x.
+    Input planes enter an LDS ring of 2s+1 planes (with a low-side halo of
+    2*K*s in y and z); level j = 1..K computes its plane t from planes
+    t-2s..t of level j-1 as soon as plane t of level j-1 exists, so K steps
+    are pipelined along x and only level K is written to HBM.  With the
+    drifting frame every dependency points to lower indices, so the pipeline
+    starts 2*K*s planes before the chunk.
+    """
+
+    name = "stream3d"
+
+    def __init__(self, ty, tz, cx, k):
+        self.ty, self.tz, self.cx, self.k = ty, tz, cx, k
+
+    def kernels(self, em, plan):
+        s = plan.sleeve
+        R = 2 * s + 1
+        fields = em.fields
+        params = ", ".join("const %s* __restrict__ lds%d" % (ct, i) for i, (n, ct) in enumerate(fields))
+        outs = ", ".join("%s& o%d" % (ct, i) for i, (n, ct) in enumerate(fields))
+        body_lines = []
+        for k in plan.stores:
+            node = plan.mm[k]
+            base = plan.rng[k].offset - 2 * s
+
+            def load(field, c, base=base):
+                fi = em.field_index[field]
+                dx, dy, dz = c[0] + base, c[1] + base, c[2] + base
+                return "lds%d[pl[%d] + o + (%d)*RZ + (%d)]" % (fi, dx + 2 * s, dy, dz)
+
+            def store(fname, v):
+                return "o%d = %s;" % (em.field_index[fname], v)
+
+            b = Body(node, load, lambda *a: None, lambda ax: None, store, "s%d_" % k)
+            body_lines.append("{")
+            body_lines += ["  " + l for l in b.lines]
+            body_lines.append("}")
+        out = ["template <int RZ>\n__device__ __forceinline__ void pt_step3(%s, const int* pl, const int o, %s) {\n%s\n}\n"
+               % (params, outs, "\n".join("  " + l for l in body_lines))]
+        nf = len(fields)
+        lds = "\n".join("  __shared__ %s B%d[K][R * PLANE];" % (ct, i) for i, (n, ct) in enumerate(fields))
+        wrap = [em.wrap_fn(a, 2 * self.k * s + 1) for a in range(3)]
+        load_plane = "\n".join("      B%d[0][slot * PLANE + idx] = FLD%d(st_in)[g];" % (i, i) for i in range(nf))
+        srcs = ", ".join("B%d[j - 1]" % i for i in range(nf))
+        decl_o = " ".join("%s o%d;" % (ct, i) for i, (n, ct) in enumerate(fields))
+        refs_o = ", ".join("o%d" % i for i in range(nf))
+        st_lds = " ".join("B%d[j][slot * PLANE + o] = o%d;" % (i, i) for i in range(nf))
+        st_glb = " ".join("FLD%d(st_out)[g] = o%d;" % (i, i) for i in range(nf))
+        out.append("""
+template <int K>
+__global__ void __launch_bounds__(256) k_step_stream(const char* __restrict__ st_in, char* __restrict__ st_out) {
+  constexpr int TY = %(TY)d, TZ = %(TZ)d, CX = %(CX)d, R = %(R)d;
+  constexpr int H = 2 * K * SLEEVE, RY = TY + H, RZ = TZ + H, PLANE = RY * RZ;
+  constexpr int NTY = N2 / TY, NTZ = N3 / TZ, NCX = N1 / CX, NT = NTY * NTZ * NCX;
+  // levels 0..K-1 live in LDS (level K goes to HBM)
+%(lds)s
+  const int b = blockIdx.x;
+  const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
+  const int cz = tile %% NTZ, cy = (tile / NTZ) %% NTY, cxi = tile / (NTZ * NTY);
+  const int x0 = cxi * CX, y0 = cy * TY, z0 = cz * TZ;
+  for (int t = x0 - H; t < x0 + CX; ++t) {
+    const int slot = ((t %% R) + R) %% R;
+    {
+      const long long gx = (long long)%(w0)s(t) * N2;
+      for (int idx = threadIdx.x; idx < PLANE; idx += 256) {
+        const int ly = idx / RZ, lz = idx - ly * RZ;
+        const long long g = (gx + %(w1)s(y0 - H + ly)) * N3 + %(w2)s(z0 - H + lz);
+%(load_plane)s
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 1; j <= K; ++j) {
+      if (t >= x0 - 2 * (K - j) * SLEEVE) {
+        int pl[R];
+#pragma unroll
+        for (int d = 0; d < R; ++d) pl[d] = ((((t - 2 * SLEEVE + d) %% R) + R) %% R) * PLANE;
+        const int lo = 2 * j * SLEEVE, W = RZ - lo, HH = RY - lo;
+        for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
+          const int rr = idx / W;
+          const int ly = lo + rr, lz = lo + (idx - rr * W);
+          const int o = ly * RZ + lz;
+          %(decl_o)s
+          pt_step3<RZ>(%(srcs)s, pl, o, %(refs_o)s);
+          if (j < K) {
+            %(st_lds)s
+          } else {
+            const long long g = ((long long)t * N2 + (y0 + ly - H)) * N3 + (z0 + lz - H);
+            %(st_glb)s
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+""" % dict(TY=self.ty, TZ=self.tz, CX=self.cx, R=R, lds=lds, w0=wrap[0], w1=wrap[1], w2=wrap[2],
+           load_plane=load_plane, srcs=srcs, decl_o=decl_o, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb))
+        return out
+
+    def launch(self, em, plan):
+        nt = (em.N[0] // self.cx) * (em.N[1] // self.ty) * (em.N[2] // self.tz)
+        return ("static void run_step_stream(int k, const char* in, char* out, hipStream_t s) {\n"
+                "  switch (k) {\n%s  default: fmr_rt::fatal(\"bad step count\");\n  }\n"
+                "  FMR_CHECK(hipGetLastError());\n}\n"
+                % "".join("  case %d: hipLaunchKernelGGL(k_step_stream<%d>, dim3(%d), dim3(256), 0, s, in, out); break;\n"
+                          % (k, k, nt) for k in range(1, self.k + 1)))
+
+    def advance(self, em):
+        return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
+                "  (void)n0;\n"
+                "  while (steps > 0) {\n"
+                "    const int k = steps >= %d ? %d : (int)steps;\n"
+                "    run_step_stream(k, g_rt.cur, g_rt.nxt, g_rt.stream); g_rt.swap();\n"
+                "    g_rt.launches += 1; steps -= k;\n  }\n}" % (self.k, self.k))
+
+
+def _stream3d_ok(em, ty, tz, cx, k):
+    plan = em.step
+    if em.dim != 3 or plan.temps:
+        return False
+    if any(m.inst[0] == "LoadIndex" for kk in plan.stores for m in plan.mm[kk].micro):
+        return False
+    if em.N[0] % cx or em.N[1] % ty or em.N[2] % tz or cx < 1:
+        return False
+    s = plan.sleeve
+    if 2 * k * s >= min(em.N):
+        return False
+    plane = (ty + 2 * k * s) * (tz + 2 * k * s)
+    lds = sum(8 if ct == "double" else 4 for _, ct in em.fields) * k * (2 * s + 1) * plane
+    return lds <= 64 * 1024
+
+
 def _tile2d_ok(em, tx, ty, k):
     plan = em.step
     if em.dim != 2 or plan.temps:
@@ -513,20 +649,77 @@ def _tile2d_ok(em, tx, ty, k):
     return lds <= 64 * 1024
 
 
-def choose_schedule(em, want: str):
-    opts = em.cfg.extra
+def schedule_from_spec(em, spec: dict):
+    """Build a schedule from an explicit spec ({"kind": ..., tile sizes, "k": ...})."""
+    kind = spec.get("kind")
+    if kind == "flat":
+        return FlatSchedule()
+    if kind == "tile2d":
+        args = (int(spec["tx"]), int(spec["ty"]), int(spec["k"]))
+        if not _tile2d_ok(em, *args):
+            raise EmitError("tile2d %r does not apply to %s" % (spec, em.name))
+        return Tile2DSchedule(*args)
+    if kind == "stream3d":
+        args = (int(spec["ty"]), int(spec["tz"]), int(spec["cx"]), int(spec["k"]))
+        if not _stream3d_ok(em, *args):
+            raise EmitError("stream3d %r does not apply to %s" % (spec, em.name))
+        return Stream3DSchedule(*args)
+    raise EmitError("unknown schedule kind %r" % (kind,))
+
+
+def candidate_specs(em) -> List[dict]:
+    """Every schedule the autotuner tries for this program (valid ones only)."""
+    out = [{"kind": "flat"}]
+    if em.dim == 2:
+        for tx, ty in ((16, 64), (32, 32), (32, 64), (16, 128), (8, 128), (64, 64), (64, 32), (16, 32)):
+            for k in (1, 2, 3, 4, 6, 8):
+                if _tile2d_ok(em, tx, ty, k):
+                    out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k})
+    if em.dim == 3:
+        for ty, tz in ((16, 64), (8, 64), (32, 32), (16, 32), (8, 32), (4, 64), (4, 32), (8, 16), (4, 16)):
+            for cx in (8, 16, 32, 64):
+                for k in (1, 2, 3, 4):
+                    nwg = (em.N[0] // cx) * (em.N[1] // ty) * (em.N[2] // tz) if _stream3d_ok(em, ty, tz, cx, k) else 0
+                    if nwg >= 128 and (nwg <= 4096 or cx >= 32):
+                        out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": k})
+    return out
+
+
+def _tuned_spec(name: str):
+    import os
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "%s.json" % name)
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("best")
+    return None
+
+
+def choose_schedule(em, want):
+    """``want``: a spec dict, "flat"/"tile2d"/"stream3d", or "auto" (tuning file, then defaults)."""
+    if isinstance(want, dict):
+        return schedule_from_spec(em, want)
     if want == "flat":
         return FlatSchedule()
-    tile = opts.get("hip_tile", [32, 32])
-    k = int(opts.get("hip_steps_per_launch", 4))
-    if want in ("auto", "tile2d"):
-        for tx, ty, kk in ((int(tile[0]), int(tile[1]), k), (32, 32, k), (16, 16, k), (32, 32, 1), (16, 16, 1), (8, 8, 1)):
-            if _tile2d_ok(em, tx, ty, kk):
-                return Tile2DSchedule(tx, ty, kk)
-        if want == "tile2d":
-            raise EmitError("the tile2d schedule does not apply to this program")
-        return FlatSchedule()
-    raise EmitError("unknown schedule %r" % want)
+    spec = em.cfg.extra.get("hip_schedule")
+    if spec is None and want == "auto":
+        spec = _tuned_spec(em.name)
+    if spec is not None:
+        try:
+            return schedule_from_spec(em, spec)
+        except EmitError:
+            if want != "auto":
+                raise
+    if want in ("auto", "tile2d") and em.dim == 2:
+        for tx, ty, k in ((32, 32, 4), (16, 16, 4), (32, 32, 1), (16, 16, 1), (8, 8, 1)):
+            if _tile2d_ok(em, tx, ty, k):
+                return Tile2DSchedule(tx, ty, k)
+    if want == "stream3d" and em.dim == 3:
+        for c in ((16, 64, 32, 1), (8, 32, 16, 1), (4, 8, 8, 1)):
+            if _stream3d_ok(em, *c):
+                return Stream3DSchedule(*c)
+    if want in ("tile2d", "stream3d"):
+        raise EmitError("the %s schedule does not apply to this program" % want)
+    return FlatSchedule()
 
 
 # --- driver --------------------------------------------------------------------------
