@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _run_pair(name, oracle_name, seed_state, steps, resident=False):
-    lay = layout_of(name)
+    lay = layout_of(oracle_name)
     o = oracle_lib(oracle_name, lay)
     h = hip_lib(name)
     for lib in (o, h):
@@ -89,3 +89,42 @@ def test_resident_multi_step_launches(name, steps):
     """Formura_Forward_Steps runs several steps per launch (temporal blocking in LDS)."""
     o, h = _run_pair(name, name, _gs_seed, steps, resident=True)
     _assert_same(o, h)
+
+
+@pytest.mark.parametrize("name,oracle,steps,resident", [
+    ("heat3d_32_stream", "heat3d_32", 11, False),     # stream3d, 1 step per launch
+    ("heat3d_32_stream", "heat3d_32", 23, True),      # stream3d, 2 steps per launch + remainder
+    ("heat3d_64_tb_stream", "heat3d_64_tb", 7, True),  # 4 steps per Forward, 3 per launch
+    ("gs_64_flat", "gs_64", 33, False),
+    ("gs_64_tile_k8", "gs_64", 61, True),             # 8 steps per launch, halo 16 of a 64 grid
+])
+def test_forced_schedules_bit_exact(name, oracle, steps, resident):
+    seed = (lambda lib: inputs.heat_field(13, lib.grid)) if name.startswith("heat") else _gs_seed
+    o, h = _run_pair(name, oracle, seed, steps, resident=resident)
+    _assert_same(o, h)
+
+
+@pytest.mark.parametrize("name,oracle,steps", [("heat3d_256", "heat3d_256", 6)])
+def test_full_size_tuned_schedule_bit_exact(name, oracle, steps):
+    o, h = _run_pair(name, oracle, lambda lib: inputs.heat_field(17, lib.grid), steps, resident=True)
+    _assert_same(o, h)
+
+
+def test_heat3d_512_properties():
+    """512^3 (beyond the oracle's reach in a test): a constant field is invariant
+    bit-exactly and the sum is conserved to rounding after many launches."""
+    h = hip_lib("heat3d_512")
+    h.init()
+    q = inputs.uniform(21, h.grid)
+    h.set_state({"q": q})
+    s0 = float(q.sum())
+    h.set_resident(True)
+    h.upload()
+    h.forward_steps(20)
+    h.download()
+    h.set_resident(False)
+    assert abs(float(h.field("q").sum()) - s0) < 1e-11 * s0
+    h.set_state({"q": np.full(h.grid, 0.7)})
+    h.forward(3)
+    assert np.all(h.field("q") == 0.7)
+    assert h.navi.offset_x == (-23) % 512
