@@ -42,10 +42,16 @@ PROGRAMS: Dict[str, tuple] = {
     "gs_64": ("grayscott2d.fmr", "gs_64.yaml", "auto"),
     "gs_2048": ("grayscott2d.fmr", "gs_2048.yaml", "auto"),
     "gs_8192": ("grayscott2d.fmr", "gs_8192.yaml", "auto"),
+    "euler3d_16": ("euler3d.fmr", "euler3d_16.yaml", "auto"),
+    "euler3d_32": ("euler3d.fmr", "euler3d_32.yaml", "auto"),
+    "euler3d_256": ("euler3d.fmr", "euler3d_256.yaml", "auto"),
     # fixed schedules, so that every schedule is parity-tested at small sizes
     "heat3d_32_stream": ("heat3d.fmr", "heat3d_32.yaml", {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 2}),
     "heat3d_64_tb_stream": ("heat3d.fmr", "heat3d_64_tb.yaml", {"kind": "stream3d", "ty": 4, "tz": 16, "cx": 16, "k": 3}),
     "gs_64_flat": ("grayscott2d.fmr", "gs_64.yaml", "flat"),
+    "euler3d_16_inline": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1}),
+    "euler3d_32_inline_stream": ("euler3d.fmr", "euler3d_32.yaml",
+                                 {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 1, "inline": 1}),
     "gs_64_tile_k8": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 8}),
 }
 

@@ -50,7 +50,21 @@ class LibEmitter:
         self.options = dict(options or {})
         if self.cfg.mpi_shape is not None:
             raise EmitError("mpi_shape is set: the HIP backend runs one rank on one GPU; omit mpi_shape")
-        self.step = GraphPlan(mm.step, self.cfg.sleeve, self.dim, "step")
+        step_mm = mm.step
+        spec = schedule if isinstance(schedule, dict) else self.cfg.extra.get("hip_schedule")
+        if schedule == "auto" and spec is None:
+            from .emit_hip import _tuned_spec
+            spec = _tuned_spec(name)
+        self.inlined = bool(isinstance(spec, dict) and spec.get("inline"))
+        if self.inlined:
+            from .manifest import inline_temporaries
+            orig_rng = calc_range(mm.step)
+            step_mm = inline_temporaries(mm.step)
+            new_rng = calc_range(step_mm)
+            for k in step_mm:
+                if (orig_rng[k].lower, orig_rng[k].upper) != (new_rng[k].lower, new_rng[k].upper):
+                    raise EmitError("inlining changed the stencil range of %s" % step_mm[k].name)
+        self.step = GraphPlan(step_mm, self.cfg.sleeve, self.dim, "step")
         self.filter = GraphPlan(mm.filter, self.cfg.filter_sleeve, self.dim, "filter") if mm.filter else None
         self.first = GraphPlan(mm.first_step, self.cfg.sleeve0, self.dim, "first_step") if mm.first_step else None
         if self.cfg.blocking is not None:
@@ -670,6 +684,20 @@ def schedule_from_spec(em, spec: dict):
 def candidate_specs(em) -> List[dict]:
     """Every schedule the autotuner tries for this program (valid ones only)."""
     out = [{"kind": "flat"}]
+    if em.mm.step and any(n.micro[-1].inst[0] != "Store" for n in em.mm.step.values()):
+        # programs with manifest temporaries: also try recomputing them where read
+        from .manifest import inline_temporaries
+        out.append({"kind": "flat", "inline": 1})
+        if em.dim == 3:
+            bpc = sum(8 if ct == "double" else 4 for _, ct in em.fields)
+            s = em.cfg.sleeve
+            for ty, tz, cx in ((8, 32, 16), (8, 32, 32), (4, 64, 16), (8, 16, 32), (16, 32, 16), (4, 32, 32),
+                               (4, 16, 32), (8, 8, 32)):
+                if (em.N[1] % ty == 0 and em.N[2] % tz == 0 and em.N[0] % cx == 0
+                        and bpc * (2 * s + 1) * (ty + 2 * s) * (tz + 2 * s) <= 64 * 1024
+                        and (em.N[0] // cx) * (em.N[1] // ty) * (em.N[2] // tz) >= 128):
+                    out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": 1, "inline": 1})
+        return out
     if em.dim == 2:
         for tx, ty in ((16, 64), (32, 32), (32, 64), (16, 128), (8, 128), (64, 64), (64, 32), (16, 32)):
             for k in (1, 2, 3, 4, 6, 8):

@@ -49,3 +49,33 @@ def grayscott_perturb(u: np.ndarray, v: np.ndarray, seed: int) -> dict:
     u2[mask] += 0.01 * r[..., 0][mask]
     v2[mask] += 0.01 * r[..., 1][mask]
     return {"u": u2, "v": v2}
+
+
+EULER_GAMMA = 1.4
+
+
+def euler_primitive(seed: int, shape):
+    """Seeded compressible-flow state: rho = 1 + 0.1 U[-1,1), p = 1 + 0.1 U[-1,1),
+    u, v, w ~ N(0, 0.1^2) (Box-Muller on independent SplitMix64 streams)."""
+    U = [uniform(seed * 16 + k, shape) for k in range(6)]
+    rho = 1.0 + 0.1 * (2.0 * U[0] - 1.0)
+    p = 1.0 + 0.1 * (2.0 * U[1] - 1.0)
+    r1 = np.sqrt(-2.0 * np.log1p(-U[2]))
+    u = 0.1 * r1 * np.cos(2.0 * np.pi * U[3])
+    v = 0.1 * r1 * np.sin(2.0 * np.pi * U[3])
+    w = 0.1 * np.sqrt(-2.0 * np.log1p(-U[4])) * np.cos(2.0 * np.pi * U[5])
+    return rho, u, v, w, p
+
+
+def euler_state(seed: int, shape) -> dict:
+    """Conserved fields r, mx, my, mz, e of ``euler_primitive``."""
+    rho, u, v, w, p = euler_primitive(seed, shape)
+    e = p / (EULER_GAMMA - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return {"r": rho, "mx": rho * u, "my": rho * v, "mz": rho * w, "e": e}
+
+
+def euler_cfl_dt(seed: int, shape, dx: float, cfl: float = 0.4) -> float:
+    """dt = CFL * dx / max(|u| + |v| + |w| + 3c) over the initial state (unsplit 3-D bound)."""
+    rho, u, v, w, p = euler_primitive(seed, shape)
+    c = np.sqrt(EULER_GAMMA * p / rho)
+    return float(cfl * dx / np.max(np.abs(u) + np.abs(v) + np.abs(w) + 3.0 * c))

@@ -169,3 +169,66 @@ def gen_mm_program(om: OMProgram) -> MMProgram:
         filter=manifest_graph(om.filter_graph, d) if om.filter_graph else None,
         step=manifest_graph(om.step_graph, d),
     )
+
+
+def inline_temporaries(mm: Dict[int, MMNode]) -> Dict[int, MMNode]:
+    """Replace every ``LoadCursor(c, X)`` of the Store nodes by X's own micro DAG
+    with its cursors shifted by ``c`` (recursively), sharing identical
+    instructions.  A manifest temporary holds, at each input position, the
+    value of its DAG at that position (Templates.hs:442-474 index it so), so
+    recomputing it where it is read gives the same numbers; the Store nodes of
+    the result read only the state fields.  Returns the Store nodes only.
+    """
+    out = {}
+    for k in sorted(mm):
+        node = mm[k]
+        if node.micro[-1].inst[0] != "Store":
+            continue
+        new: List[Micro] = []
+        memo: Dict[tuple, int] = {}
+
+        def emit(inst, typ):
+            key = (inst, typ)
+            if key not in memo:
+                memo[key] = len(new)
+                new.append(Micro(inst, typ))
+            return memo[key]
+
+        def copy(src: MMNode, shift, j, cache):
+            if j in cache:
+                return cache[j]
+            m = src.micro[j]
+            k0 = m.inst[0]
+            if k0 == "LoadCursorStatic":
+                r = emit(("LoadCursorStatic", _add(m.inst[1], shift), m.inst[2]), m.typ)
+            elif k0 == "LoadCursor":
+                tgt = mm[m.inst[2]]
+                r = copy(tgt, _add(m.inst[1], shift), len(tgt.micro) - 1, {})
+            elif k0 == "Imm" or k0 == "LoadIndex":
+                r = emit(m.inst, m.typ)
+            elif k0 == "Uniop":
+                r = emit(("Uniop", m.inst[1], copy(src, shift, m.inst[2], cache)), m.typ)
+            elif k0 == "Binop":
+                a = copy(src, shift, m.inst[2], cache)
+                b = copy(src, shift, m.inst[3], cache)
+                r = emit(("Binop", m.inst[1], a, b), m.typ)
+            elif k0 == "Triop":
+                a = copy(src, shift, m.inst[2], cache)
+                b = copy(src, shift, m.inst[3], cache)
+                c = copy(src, shift, m.inst[4], cache)
+                r = emit(("Triop", m.inst[1], a, b, c), m.typ)
+            elif k0 == "Store":
+                r = emit(("Store", m.inst[1], copy(src, shift, m.inst[2], cache)), m.typ)
+            else:
+                raise ValueError(k0)
+            cache[j] = r
+            return r
+
+        dim = None
+        for m in node.micro:
+            if m.inst[0] in ("LoadCursorStatic", "LoadCursor"):
+                dim = len(m.inst[1])
+                break
+        copy(node, (0,) * (dim or 3), len(node.micro) - 1, {})
+        out[k] = MMNode(k, new, node.typ, node.name)
+    return out

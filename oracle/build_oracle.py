@@ -39,7 +39,13 @@ CONFIGS = {
     "heat3d_256": ("heat3d_ref.c", {"NX": 256}),
     "gs_64": ("grayscott2d_ref.c", {"NX": 64}),
     "gs_2048": ("grayscott2d_ref.c", {"NX": 2048}),
+    "euler3d_16": ("euler3d_ref.c", {"NX": 16}),
+    "euler3d_32": ("euler3d_ref.c", {"NX": 32}),
+    "euler3d_256": ("euler3d_ref.c", {"NX": 256}),
 }
+
+# static arrays beyond 2 GiB need the medium code model
+LARGE = {"euler3d_256"}
 
 
 def lib_path(name: str, kind: str = "strict") -> str:
@@ -49,6 +55,8 @@ def lib_path(name: str, kind: str = "strict") -> str:
 def build(name: str, kind: str = "strict", force: bool = False) -> str:
     src, defs = CONFIGS[name]
     flags = (STRICT if kind == "strict" else FAST) + ["-D%s=%s" % kv for kv in sorted(defs.items())]
+    if name in LARGE:
+        flags.append("-mcmodel=medium")
     srcp = os.path.join(HERE, src)
     out = lib_path(name, kind)
     stamp = hashlib.sha1((" ".join(flags) + open(srcp).read()).encode()).hexdigest()

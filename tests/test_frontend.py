@@ -160,3 +160,44 @@ def test_reference_programs_compile(fmr, yml, sleeve, manifest):
     mm = gen_mm_program(om)
     assert om.config.sleeve == sleeve
     assert len(mm.step) == manifest
+
+
+def test_euler_frontend_graph_matches_oracle_bit_exact():
+    """The HLLC program: 15 manifest flux temporaries + 5 stores, through the
+    numpy interpreter of the front-end's graphs, against the oracle."""
+    name = "euler3d_16"
+    mm = _mm(name)
+    assert len([n for n in mm.step.values() if n.name.endswith("_next")]) == 5
+    assert len(mm.step) == 20
+    o = oracle_lib(name, layout_of(name))
+    o.init()
+    st = inputs.euler_state(2, o.grid)
+    o.set_state(st)
+    fields = {k: v.copy() for k, v in st.items()}
+    for _ in range(3):
+        fields = mm_interp.step(mm.step, mm.om.config.sleeve, fields)
+    o.forward(3)
+    for f in fields:
+        np.testing.assert_array_equal(fields[f], o.field(f))
+
+
+def test_inlined_temporaries_match_oracle_bit_exact():
+    """Recomputing the manifest temporaries where they are read (the fused GPU
+    schedules) gives the same numbers as materialising them."""
+    from formura_amd.manifest import inline_temporaries
+    name = "euler3d_16"
+    mm = _mm(name)
+    inl = inline_temporaries(mm.step)
+    assert len(inl) == 5 and all(m.inst[0] != "LoadCursor" for n in inl.values() for m in n.micro)
+    assert {k: (r.lower, r.upper) for k, r in calc_range(inl).items()} == \
+        {k: (r.lower, r.upper) for k, r in calc_range(mm.step).items() if k in inl}
+    o = oracle_lib(name, layout_of(name))
+    o.init()
+    st = inputs.euler_state(6, o.grid)
+    o.set_state(st)
+    fields = {k: v.copy() for k, v in st.items()}
+    for _ in range(2):
+        fields = mm_interp.step(inl, mm.om.config.sleeve, fields)
+    o.forward(2)
+    for f in fields:
+        np.testing.assert_array_equal(fields[f], o.field(f))

@@ -128,3 +128,23 @@ def test_heat3d_512_properties():
     h.forward(3)
     assert np.all(h.field("q") == 0.7)
     assert h.navi.offset_x == (-23) % 512
+
+
+def _euler_seed(lib):
+    return inputs.euler_state(1, lib.grid)
+
+
+@pytest.mark.parametrize("name,oracle,steps,resident", [
+    ("euler3d_16", "euler3d_16", 10, False),                 # 15 flux-temporary kernels + store kernel
+    ("euler3d_32", "euler3d_32", 5, True),
+    ("euler3d_16_inline", "euler3d_16", 8, False),           # temporaries recomputed in one kernel
+    ("euler3d_32_inline_stream", "euler3d_32", 6, True),     # ... and streamed through LDS
+])
+def test_euler_hllc_bit_exact(name, oracle, steps, resident):
+    o, h = _run_pair(name, oracle, _euler_seed, steps, resident=resident)
+    _assert_same(o, h)
+
+
+def test_euler_hllc_full_size():
+    o, h = _run_pair("euler3d_256", "euler3d_256", _euler_seed, 2, resident=True)
+    _assert_same(o, h)
