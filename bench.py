@@ -82,6 +82,48 @@ def run_gpu(name: str, steps: int, warmup: int, bytes_per_lup: float, seed: int 
     }
 
 
+def run_ensemble(count: int, steps: int):
+    """The batched FP32 (Kahan) Gray-Scott ensemble and its FP64 twin on the GPU.
+
+    Returns GLUPS of both (all instances, whole ensemble, state resident) and the
+    worst relative L2 error of any instance/field of the FP32 run against its own
+    FP64 run."""
+    import numpy as np
+    from formura_amd import batch as B
+    sizes, params, seeds = B.grayscott_ensemble(count)
+    inits = [B.grayscott_init(s[0], sd) for s, sd in zip(sizes, seeds)]
+    cells = sum(a * b for a, b in sizes)
+    out = {"instances": count, "cells": cells, "steps": steps,
+           "sides": [min(s[0] for s in sizes), max(s[0] for s in sizes)]}
+    ens = {}
+    for name in ("gs_batch_f32k", "gs_batch_f64"):
+        e = B.Ensemble(name, sizes, params)
+        for i, st in enumerate(inits):
+            e.set_state(i, st)
+        e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"])  # warm-up, captures the graph
+        for i, st in enumerate(inits):
+            e.set_state(i, st)
+        ms = e.step(steps)
+        out[name] = {"glups": cells * steps / (ms * 1e-3) / 1e9, "ms": ms,
+                     "tile": e.desc["tile"], "steps_per_launch": e.desc["steps_per_launch"]}
+        ens[name] = e
+    errs, rms = [], {"u": 0.0, "v": 0.0}
+    for i in range(count):
+        x, y = ens["gs_batch_f64"].get_state(i), ens["gs_batch_f32k"].get_state(i)
+        num = sum(float(np.sum((y[f] - x[f]) ** 2)) for f in ("u", "v"))
+        den = sum(float(np.sum(x[f] ** 2)) for f in ("u", "v"))
+        errs.append((num / den) ** 0.5)
+        for f in ("u", "v"):
+            rms[f] = max(rms[f], float(np.sqrt(np.mean((y[f] - x[f]) ** 2))))
+    errs = np.array(errs)
+    out["f32k_rel_l2_vs_f64"] = {"max": float(errs.max()), "median": float(np.median(errs)),
+                                 "instances_le_1e-5": int((errs <= 1e-5).sum())}
+    out["f32k_max_rms_error_per_field"] = rms
+    for e in ens.values():
+        e.close()
+    return out
+
+
 def cpu_baseline(name: str, target_s: float = 10.0):
     """Oracle (fast build) on the host cores, timed on a bounded number of steps."""
     import numpy as np
@@ -123,6 +165,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--ensemble", type=int, default=1024,
+                    help="instances of the batched FP32 Gray-Scott ensemble (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB, 20 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -167,6 +212,13 @@ def main():
     }
     if not r["finite"]:
         line["error"] = "non-finite state after the run"
+    if args.ensemble > 0:
+        try:
+            ens = run_ensemble(args.ensemble, args.ensemble_steps)
+            ens["fp32_batched_vs_plain_fp64"] = ens["gs_batch_f32k"]["glups"] / r["glups"]
+            line["ensemble"] = ens
+        except Exception as e:
+            line["ensemble"] = {"error": str(e)[:300]}
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         try:
             line["cpu_baseline"] = cpu_baseline(name, args.cpu_seconds)

@@ -56,6 +56,73 @@ PROGRAMS: Dict[str, tuple] = {
 }
 
 
+# batched ensembles: name -> (program, config, precision, lifted literals, tile x, tile y, steps per launch)
+BATCHES: Dict[str, tuple] = {
+    "gs_batch_f64": ("grayscott2d.fmr", "gs_64.yaml", "f64", ["f", "k"], 16, 32, 3),
+    "gs_batch_f32k": ("grayscott2d.fmr", "gs_64.yaml", "f32k", ["f", "k"], 16, 32, 4),
+}
+
+
+def build_batch(name: str, force: bool = False, spec: Optional[dict] = None, out_dir: Optional[str] = None) -> str:
+    from .emit_batch import BatchEmitter
+    from .emit_hip import generate
+    fmr, yml, prec, params, tx, ty, k = BATCHES[name]
+    if spec is None:
+        spec = _tuned_batch(name)
+    if spec:
+        tx, ty, k = int(spec["tx"]), int(spec["ty"]), int(spec["k"])
+    _, _, mm = generate(os.path.join(PROG_DIR, fmr), os.path.join(PROG_DIR, yml), name, "flat")
+    src = BatchEmitter(mm, name, prec, params, tx, ty, k).emit()
+    d = out_dir or os.path.join(BUILD_DIR, name)
+    os.makedirs(d, exist_ok=True)
+    p = os.path.join(d, "%s.hip" % name)
+    if not os.path.exists(p) or open(p).read() != src:
+        with open(p, "w") as f:
+            f.write(src)
+    hdr = os.path.join(d, "%s.h" % name)
+    if not os.path.exists(hdr):
+        open(hdr, "w").write("/* batch library: see include/formura_batch.h */\n")
+    return compile_library(name, d, force=force)
+
+
+# adjoint libraries: name -> (program, config, lifted parameters, real type)
+ADJOINTS: Dict[str, tuple] = {
+    "gs_64_adj": ("grayscott2d.fmr", "gs_64.yaml", ["f", "k", "ru", "rv"], "double"),
+    "gs_64_adj_f32": ("grayscott2d.fmr", "gs_64.yaml", ["f", "k", "ru", "rv"], "float"),
+    "gs_1024_adj": ("grayscott2d.fmr", "gs_1024.yaml", ["f", "k", "ru", "rv"], "double"),
+    "gs_1024_adj_f32": ("grayscott2d.fmr", "gs_1024.yaml", ["f", "k", "ru", "rv"], "float"),
+    "heat3d_16_adj": ("heat3d.fmr", "heat3d_16.yaml", [], "double"),
+    "heat3d_256_adj": ("heat3d.fmr", "heat3d_256.yaml", [], "double"),
+}
+
+
+def build_adjoint(name: str, force: bool = False) -> str:
+    from .emit_adjoint import AdjointEmitter
+    from .emit_hip import generate
+    fmr, yml, params, real = ADJOINTS[name]
+    _, _, mm = generate(os.path.join(PROG_DIR, fmr), os.path.join(PROG_DIR, yml), name, "flat")
+    src = AdjointEmitter(mm, name, params, real).emit()
+    d = os.path.join(BUILD_DIR, name)
+    os.makedirs(d, exist_ok=True)
+    p = os.path.join(d, "%s.hip" % name)
+    if not os.path.exists(p) or open(p).read() != src:
+        with open(p, "w") as f:
+            f.write(src)
+    hdr = os.path.join(d, "%s.h" % name)
+    if not os.path.exists(hdr):
+        open(hdr, "w").write("/* adjoint library: see include/formura_adjoint.h */\n")
+    return compile_library(name, d, force=force)
+
+
+def _tuned_batch(name: str):
+    p = os.path.join(PKG, "tuning", "%s.json" % name)
+    if os.path.exists(p):
+        import json
+        with open(p) as f:
+            return json.load(f).get("best")
+    return None
+
+
 def lib_path(name: str) -> str:
     return os.path.join(BUILD_DIR, name, "lib%s.so" % name)
 
@@ -81,7 +148,8 @@ def compile_library(name: str, src_dir: str, force: bool = False) -> str:
     src = os.path.join(src_dir, "%s.hip" % name)
     out = os.path.join(src_dir, "lib%s.so" % name)
     deps = [src, os.path.join(src_dir, "%s.h" % name), os.path.join(CSRC, "formura_hip_rt.h"),
-            os.path.join(INCLUDE, "formura_hip.h")]
+            os.path.join(INCLUDE, "formura_hip.h"), os.path.join(INCLUDE, "formura_batch.h"),
+            os.path.join(INCLUDE, "formura_adjoint.h")]
     stamp = hashlib.sha1(("\0".join(HIP_FLAGS) + "".join(open(p).read() for p in deps)).encode()).hexdigest()
     stamp_file = out + ".stamp"
     if not force and os.path.exists(out) and os.path.exists(stamp_file) and open(stamp_file).read() == stamp:
@@ -103,11 +171,12 @@ def build_one(name: str, force: bool = False) -> str:
 
 
 def build_all(names: Optional[List[str]] = None, force: bool = False, jobs: int = 0) -> List[str]:
-    names = names or list(PROGRAMS)
+    names = names or (list(PROGRAMS) + list(BATCHES) + list(ADJOINTS))
     jobs = jobs or min(8, os.cpu_count() or 4)
     outs = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = {ex.submit(build_one, n, force): n for n in names}
+        futs = {ex.submit(build_batch if n in BATCHES else (build_adjoint if n in ADJOINTS else build_one), n, force): n
+                for n in names}
         for f in cf.as_completed(futs):
             outs.append(f.result())
     return outs

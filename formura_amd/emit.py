@@ -183,16 +183,44 @@ class Body:
     """C statements for one manifest node's micro DAG."""
 
     def __init__(self, node: MMNode, load: Callable, tmp_load: Callable, index: Callable,
-                 store: Callable, prefix: str, fold: bool = True, const_kw="const "):
+                 store: Callable, prefix: str, fold: bool = True, const_kw="const ",
+                 params: Dict[str, str] = None, real: str = None):
+        """``params``: literal names bound to run-time expressions (not folded);
+        ``real``: if set ("float"), every floating micro value is computed in
+        that type (the FP32 form; constants are folded in double and rounded,
+        and a division by a constant becomes a multiplication by its reciprocal)."""
         self.lines: List[str] = []
         vals: Dict[int, object] = {}
+        params = params or {}
         for j, m in enumerate(node.micro):
             typ = m.typ
             ct = CTYPE.get(typ, typ)
+            if real and ct == "double":
+                ct = real
             name = "%s%d" % (prefix, j)
             k = m.inst[0]
             if k == "Store":
                 self.lines.append(store(m.inst[1], "%s%d" % (prefix, m.inst[2])))
+                continue
+            if m.src in params and k not in ("LoadCursorStatic", "LoadCursor", "LoadIndex"):
+                self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, params[m.src]))
+                continue
+            if (real == "float" and k == "Binop" and m.inst[1] == "-" and m.inst[2] not in vals
+                    and node.micro[m.inst[2]].inst[0] == "Binop" and node.micro[m.inst[2]].inst[1] == "+"
+                    and node.micro[m.inst[3]].inst[0] == "Binop" and node.micro[m.inst[3]].inst[1] == "*"
+                    and vals.get(node.micro[m.inst[3]].inst[2]) == 2.0):
+                # FP32 form: (a + b) - 2*c evaluated as (a - c) + (b - c).  For a, b
+                # close to c the differences are exact (Sterbenz), so a discrete
+                # second difference loses no digits to cancellation.
+                pa, pb = node.micro[m.inst[2]].inst[2], node.micro[m.inst[2]].inst[3]
+                pc = node.micro[m.inst[3]].inst[3]
+                self.lines.append("%s%s %s = (%s%d - %s%d) + (%s%d - %s%d);"
+                                  % (const_kw, ct, name, prefix, pa, prefix, pc, prefix, pb, prefix, pc))
+                continue
+            if real and k == "Binop" and m.inst[1] == "/" and m.inst[3] in vals and m.inst[2] not in vals:
+                rc = 1.0 / vals[m.inst[3]] if vals[m.inst[3]] else float("inf")
+                self.lines.append("%s%s %s = %s%d*%s;" % (const_kw, ct, name, prefix, m.inst[2],
+                                                        _const_literal(_f32(rc), "float")))
                 continue
             if fold:
                 ops = _operands(m)
@@ -200,11 +228,15 @@ class Body:
                     v = _fold(m, vals, typ)
                     if v is not None:
                         vals[j] = v
-                        self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, _const_literal(v, typ)))
+                        lit = _const_literal(_f32(v) if (real == "float" and isinstance(v, float)) else v,
+                                             "float" if (real == "float" and isinstance(v, float)) else typ)
+                        self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, lit))
                         continue
             a = lambda i: "%s%d" % (prefix, i)
             if k == "Imm":
                 rhs = show_double(float(Fraction(m.inst[1])))
+                if real == "float":
+                    rhs = _const_literal(_f32(float(Fraction(m.inst[1]))), "float")
             elif k == "LoadCursorStatic":
                 rhs = load(m.inst[2], m.inst[1])
             elif k == "LoadCursor":
@@ -214,13 +246,13 @@ class Body:
             elif k == "Uniop":
                 op = m.inst[1]
                 if op.startswith("external-call/"):
-                    rhs = "%s((double)%s)" % (op.split("/", 1)[1], a(m.inst[2]))
+                    rhs = "%s((%s)%s)" % (op.split("/", 1)[1], real or "double", a(m.inst[2]))
                 else:
                     rhs = "%s%s" % (op, a(m.inst[2]))
             elif k == "Binop":
                 op = m.inst[1]
                 if op == "**":
-                    rhs = "pow((double)%s,(double)%s)" % (a(m.inst[2]), a(m.inst[3]))
+                    rhs = "pow((%s)%s,(%s)%s)" % (real or "double", a(m.inst[2]), real or "double", a(m.inst[3]))
                 else:
                     rhs = "%s%s%s" % (a(m.inst[2]), op, a(m.inst[3]))
             elif k == "Triop":
@@ -229,6 +261,7 @@ class Body:
                 raise EmitError("Unimplemented for keyword: %s" % k)
             self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, rhs))
         self.result = "%s%d" % (prefix, len(node.micro) - 1)
+        self.vals = vals
 
 
 def _operands(m):

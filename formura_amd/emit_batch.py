@@ -1,0 +1,395 @@
+"""Batched ensembles of one 2-D program: many instances, uneven grids, per-instance
+parameters, one persistent launch per K steps, hipGraph-captured step loops.
+
+The generated library implements ``include/formura_batch.h``.  Every instance
+is an independent periodic run of the program with its own grid (n1 x n2, both
+multiples of the tile), its own values of the lifted literals (e.g. ``f`` and
+``k`` of Gray-Scott), and its own drifting frame (offset = -s*steps mod n).
+
+Two precisions:
+
+``f64``   the program's own arithmetic, bit-identical to single-instance runs
+          of the same program (and to the reference's C) when the parameters
+          equal the program's literals;
+``f32k``  FP32 state and FP32 arithmetic, with Kahan-compensated accumulation
+          of the time update ``q' = q + dq`` (Kahan 1965): each field carries a
+          compensation term c, updated as y = dq - c, t = q + y,
+          c' = (t - q) - y, q' = t.  The compensation moves with the cell in the
+          drifting frame and stays in LDS across the K fused steps.
+
+Kernel: the ``tile2d`` scheme (tile + low-side halo of 2*K*s in LDS, K steps,
+ping-pong buffers) over a table of tiles of all instances; a fixed-size grid of
+workgroups loops over the tile table (persistent launch).  The host captures G
+launches (an even number, so the ping-pong buffers return to their start) in
+a hipGraph and replays it.
+"""
+from __future__ import annotations
+
+import json
+from typing import Dict, List
+
+from . import syntax as S
+from .emit import Body, EmitError, GraphPlan, state_fields
+from .manifest import calc_range
+
+PRECISIONS = {"f64": ("double", False), "f32k": ("float", True)}
+
+
+class BatchEmitter:
+    def __init__(self, mm, name: str, precision: str, params: List[str], tx: int, ty: int, k: int,
+                 graph_launches: int = 16, grid_wgs: int = 2048):
+        self.mm, self.om, self.name = mm, mm.om, name
+        if self.om.dimension != 2:
+            raise EmitError("batched ensembles are built for 2-D programs")
+        if precision not in PRECISIONS:
+            raise EmitError("unknown precision %r" % precision)
+        self.real, self.kahan = PRECISIONS[precision]
+        self.precision = precision
+        self.params = list(params)
+        self.tx, self.ty, self.k = tx, ty, k
+        self.G = graph_launches + (graph_launches % 2)
+        self.grid_wgs = grid_wgs
+        self.plan = GraphPlan(mm.step, self.om.config.sleeve, 2, "step")
+        if self.plan.temps:
+            raise EmitError("batched ensembles need a program without manifest temporaries")
+        self.fields = state_fields(self.om)
+        self.fi = {n: i for i, (n, _) in enumerate(self.fields)}
+        names = set()
+        for node in self.plan.mm.values():
+            for m in node.micro:
+                if m.src:
+                    names.add(m.src)
+                if m.inst[0] == "LoadIndex":
+                    raise EmitError("index expressions are not supported in batched ensembles")
+        missing = [p for p in self.params if p not in names]
+        if missing:
+            raise EmitError("parameters %s are not literals of the step" % missing)
+
+    def emit(self) -> str:
+        s = self.plan.sleeve
+        nf = len(self.fields)
+        real = self.real
+        pmap = {p: "P[%d]" % i for i, p in enumerate(self.params)}
+        lines = []
+        for k in self.plan.stores:
+            node = self.plan.mm[k]
+            base = self.plan.rng[k].offset - 2 * s
+            fname = node.micro[-1].inst[1]
+            fi = self.fi[fname]
+
+            def load(field, c, base=base):
+                return "lds%d[o + (%d)*RY + (%d)]" % (self.fi[field], c[0] + base, c[1] + base)
+
+            body = Body(node, load, lambda *a: None, lambda ax: None, lambda n, v: "", "s%d_" % k,
+                        params=pmap, real=("float" if real == "float" else None))
+            lines.append("{")
+            lines += ["  " + l for l in body.lines]
+            if self.kahan:
+                q, dq = self._update_ids(node, fname, base)
+                lines.append("  old%d = s%d_%d; inc%d = s%d_%d;" % (fi, k, q, fi, k, dq))
+            else:
+                lines.append("  old%d = 0; inc%d = s%d_%d;" % (fi, fi, k, node.micro[-1].inst[2]))
+            lines.append("}")
+        params = ", ".join("const %s* __restrict__ lds%d" % (real, i) for i in range(nf))
+        outs = ", ".join("%s& old%d, %s& inc%d" % (real, i, real, i) for i in range(nf))
+        cbase = "(%d)*RY + (%d)" % (self.plan.rng[self.plan.stores[0]].offset - 2 * s,
+                                    self.plan.rng[self.plan.stores[0]].offset - 2 * s)
+        lds = "\n".join("  __shared__ %s B%d[2][RX * RY];" % (real, i) for i in range(nf))
+        if self.kahan:
+            lds += "\n" + "\n".join("  __shared__ %s C%d[2][RX * RY];" % (real, i) for i in range(nf))
+        load_tile = "\n".join("      B%d[0][idx] = src[%dLL * plane + g];" % (i, i) for i in range(nf))
+        if self.kahan:
+            load_tile += "\n" + "\n".join("      C%d[0][idx] = src[%dLL * plane + g];" % (i, nf + i) for i in range(nf))
+        srcs = ", ".join("B%d[sb]" % i for i in range(nf))
+        decl = " ".join("%s old%d, inc%d;" % (real, i, i) for i in range(nf))
+        refs = ", ".join("old%d, inc%d" % (i, i) for i in range(nf))
+        if self.kahan:
+            # Kahan: ky = dq - c, kt = q + ky, kc = (kt - q) - ky (names that shadow nothing)
+            upd = "\n".join(
+                "        const %(r)s ky%(i)d = inc%(i)d - C%(i)d[sb][o + CB];\n"
+                "        const %(r)s kt%(i)d = old%(i)d + ky%(i)d;\n"
+                "        const %(r)s kc%(i)d = (kt%(i)d - old%(i)d) - ky%(i)d;" % {"r": real, "i": i} for i in range(nf))
+            st_lds = " ".join("B%d[db][o] = kt%d; C%d[db][o] = kc%d;" % (i, i, i, i) for i in range(nf))
+            st_glb = " ".join("dst[%dLL * plane + g] = kt%d; dst[%dLL * plane + g] = kc%d;" % (i, i, nf + i, i)
+                              for i in range(nf))
+        else:
+            upd = "\n".join("        const %s kt%d = inc%d;" % (real, i, i) for i in range(nf))
+            st_lds = " ".join("B%d[db][o] = kt%d;" % (i, i) for i in range(nf))
+            st_glb = " ".join("dst[%dLL * plane + g] = kt%d;" % (i, i) for i in range(nf))
+        nslots = nf * (2 if self.kahan else 1)
+        desc = {"name": self.name, "precision": self.precision, "real": real, "kahan": self.kahan,
+                "fields": [n for n, _ in self.fields], "params": self.params, "sleeve": s,
+                "tile": [self.tx, self.ty], "steps_per_launch": self.k, "graph_launches": self.G,
+                "grid_wgs": self.grid_wgs, "slots_per_cell": nslots}
+        return TEMPLATE % dict(
+            name=self.name, real=real, nf=nf, nslots=nslots, np=max(1, len(self.params)), sleeve=s,
+            TX=self.tx, TY=self.ty, KMAX=self.k, G=self.G, GRID=self.grid_wgs, desc=json.dumps(desc),
+            pt_params=params, pt_outs=outs, pt_body="\n".join("  " + l for l in lines),
+            lds=lds, load_tile=load_tile, srcs=srcs, decl=decl, refs=refs, upd=upd, st_lds=st_lds,
+            st_glb=st_glb, CB=cbase, kahan=1 if self.kahan else 0,
+            preload="\n".join("  { hipFuncAttributes fa; FMR_CHECK(hipFuncGetAttributes(&fa, (const void*)k_batch<%d>)); }"
+                              % kk for kk in range(1, self.k + 1)),
+            launch_cases="".join(
+                "    case %d: hipLaunchKernelGGL(k_batch<%d>, dim3(grid), dim3(256), 0, b->stream, b->inst_d, "
+                "b->tile_inst_d, b->ntiles, in, out); break;\n" % (kk, kk) for kk in range(1, self.k + 1)))
+
+    def _update_ids(self, node, fname, base):
+        st = node.micro[-1]
+        a = node.micro[st.inst[2]]
+        if a.inst[0] != "Binop" or a.inst[1] != "+":
+            raise EmitError("the f32k form needs an update of the form q + dq (node %s)" % node.name)
+        for x, y in ((a.inst[2], a.inst[3]), (a.inst[3], a.inst[2])):
+            m = node.micro[x]
+            if (m.inst[0] == "LoadCursorStatic" and m.inst[2] == fname
+                    and all(c + base == base for c in m.inst[1])):
+                return x, y
+        raise EmitError("the f32k form needs an update q + dq with q the cell's own value (node %s)" % node.name)
+
+
+TEMPLATE = r'''// Generated by formura_amd: batched ensemble of %(name)s. Do not edit.
+#include "formura_hip_rt.h"
+#include "formura_batch.h"
+#include <vector>
+
+namespace {
+typedef %(real)s real_t;
+constexpr int NF = %(nf)d;        // state fields
+constexpr int NSLOT = %(nslots)d; // arrays per instance (fields, then compensations)
+constexpr int NP = %(np)d;        // lifted parameters
+constexpr int SLEEVE = %(sleeve)d;
+constexpr int KMAX = %(KMAX)d;
+constexpr int GRAPH_LAUNCHES = %(G)d;
+const char* const kDescription = R"FMRJSON(%(desc)s)FMRJSON";
+
+struct Inst {
+  long long base;  // element offset of the instance's arrays
+  int n1, n2, nty, tile0;
+  double p[NP];
+};
+
+template <int RY>
+__device__ __forceinline__ void pt(%(pt_params)s, const int o, const double* P, %(pt_outs)s) {
+%(pt_body)s
+}
+
+__device__ __forceinline__ int wrapn(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+template <int K>
+__global__ void __launch_bounds__(256) k_batch(const Inst* __restrict__ inst, const int* __restrict__ tile_inst,
+                                               int ntiles, const real_t* __restrict__ in, real_t* __restrict__ out) {
+  constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
+  constexpr int CB = %(CB)s;  // the cell's own value relative to the output point
+%(lds)s
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {  // persistent loop over all instances' tiles
+    const Inst I = inst[tile_inst[tile]];
+    const int lt = tile - I.tile0;
+    const int tx0 = (lt / I.nty) * TX, ty0 = (lt %% I.nty) * TY;
+    const long long plane = (long long)I.n1 * I.n2;
+    const real_t* __restrict__ src = in + I.base;
+    real_t* __restrict__ dst = out + I.base;
+    double P[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) P[q] = I.p[q];
+    for (int idx = threadIdx.x; idx < RX * RY; idx += 256) {
+      const int r = idx / RY, c = idx - r * RY;
+      const long long g = (long long)wrapn(tx0 - H + r, I.n1) * I.n2 + wrapn(ty0 - H + c, I.n2);
+%(load_tile)s
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 1; j <= K; ++j) {
+      const int sb = (j - 1) & 1, db = j & 1;
+      const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
+      for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
+        const int rr = idx / W;
+        const int r = lo + rr, c = lo + (idx - rr * W);
+        const int o = r * RY + c;
+        %(decl)s
+        pt<RY>(%(srcs)s, o, P, %(refs)s);
+%(upd)s
+        if (j < K) {
+          %(st_lds)s
+        } else {
+          const long long g = (long long)(tx0 + r - H) * I.n2 + (ty0 + c - H);
+          %(st_glb)s
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+}  // namespace
+
+struct FormuraBatch {
+  int ninst = 0, ntiles = 0;
+  std::vector<Inst> inst;
+  std::vector<long long> steps_done;
+  Inst* inst_d = nullptr;
+  int* tile_inst_d = nullptr;
+  real_t *cur = nullptr, *nxt = nullptr;
+  long long total = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // one captured graph per ping-pong parity: graph p reads buf[p] (and, with an
+  // even number of launches, leaves the state there)
+  real_t* buf[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  double last_ms = 0.0;
+  long long launches = 0;
+};
+
+static void launch_k(FormuraBatch* b, int k, const real_t* in, real_t* out) {
+  const int grid = b->ntiles < %(GRID)d ? b->ntiles : %(GRID)d;
+  switch (k) {
+%(launch_cases)s    default: fmr_rt::fatal("bad step count");
+  }
+  ++b->launches;
+}
+
+extern "C" const char* Formura_Batch_Describe(void) { return kDescription; }
+
+extern "C" FormuraBatch* Formura_Batch_Create(int ninst, const int* n1, const int* n2, const double* params) {
+  constexpr int TX = %(TX)d, TY = %(TY)d;
+  if (ninst <= 0) fmr_rt::fatal("Formura_Batch_Create: no instances");
+  FormuraBatch* b = new FormuraBatch();
+  b->ninst = ninst;
+  b->inst.resize(ninst);
+  b->steps_done.assign(ninst, 0);
+  std::vector<int> tile_inst;
+  long long base = 0;
+  for (int i = 0; i < ninst; ++i) {
+    if (n1[i] %% TX || n2[i] %% TY || n1[i] <= 2 * KMAX * SLEEVE || n2[i] <= 2 * KMAX * SLEEVE)
+      fmr_rt::fatal("Formura_Batch_Create: every grid extent must be a multiple of the tile and exceed the halo");
+    Inst& I = b->inst[i];
+    I.base = base;
+    I.n1 = n1[i];
+    I.n2 = n2[i];
+    I.nty = n2[i] / TY;
+    I.tile0 = (int)tile_inst.size();
+    for (int q = 0; q < NP; ++q) I.p[q] = params ? params[(long long)i * NP + q] : 0.0;
+    const int nt = (n1[i] / TX) * (n2[i] / TY);
+    for (int t = 0; t < nt; ++t) tile_inst.push_back(i);
+    base += (long long)NSLOT * n1[i] * n2[i];
+  }
+  b->ntiles = (int)tile_inst.size();
+  b->total = base;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) fmr_rt::fatal("no HIP device is visible");
+  b->inst_d = (Inst*)fmr_rt::device_alloc(sizeof(Inst) * ninst);
+  b->tile_inst_d = (int*)fmr_rt::device_alloc(sizeof(int) * tile_inst.size());
+  b->cur = (real_t*)fmr_rt::device_alloc(sizeof(real_t) * base);
+  b->nxt = (real_t*)fmr_rt::device_alloc(sizeof(real_t) * base);
+  b->buf[0] = b->cur;
+  b->buf[1] = b->nxt;
+  // load every kernel's code object now: the first launch of a kernel must not
+  // happen inside a stream capture (Formura_Batch_Step captures launches)
+%(preload)s
+  FMR_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  FMR_CHECK(hipEventCreate(&b->ev0));
+  FMR_CHECK(hipEventCreate(&b->ev1));
+  FMR_CHECK(hipMemcpyAsync(b->inst_d, b->inst.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, b->stream));
+  FMR_CHECK(hipMemcpyAsync(b->tile_inst_d, tile_inst.data(), sizeof(int) * tile_inst.size(), hipMemcpyHostToDevice,
+                           b->stream));
+  FMR_CHECK(hipStreamSynchronize(b->stream));
+  return b;
+}
+
+// state of one instance as NF fields of n1*n2 doubles (row-major); the
+// compensation terms start at zero
+extern "C" void Formura_Batch_Set_State(FormuraBatch* b, int i, const double* fields) {
+  if (i < 0 || i >= b->ninst) fmr_rt::fatal("Formura_Batch_Set_State: bad instance");
+  const Inst& I = b->inst[i];
+  const long long plane = (long long)I.n1 * I.n2;
+  std::vector<real_t> h((size_t)NSLOT * plane, (real_t)0);
+  for (long long e = 0; e < NF * plane; ++e) h[e] = (real_t)fields[e];
+  FMR_CHECK(hipMemcpyAsync(b->cur + I.base, h.data(), sizeof(real_t) * h.size(), hipMemcpyHostToDevice, b->stream));
+  FMR_CHECK(hipStreamSynchronize(b->stream));
+  b->steps_done[i] = 0;
+}
+
+extern "C" void Formura_Batch_Get_State(FormuraBatch* b, int i, double* fields) {
+  if (i < 0 || i >= b->ninst) fmr_rt::fatal("Formura_Batch_Get_State: bad instance");
+  const Inst& I = b->inst[i];
+  const long long plane = (long long)I.n1 * I.n2;
+  std::vector<real_t> h((size_t)NF * plane);
+  FMR_CHECK(hipMemcpyAsync(h.data(), b->cur + I.base, sizeof(real_t) * h.size(), hipMemcpyDeviceToHost, b->stream));
+  FMR_CHECK(hipStreamSynchronize(b->stream));
+  for (long long e = 0; e < NF * plane; ++e) fields[e] = (double)h[e];
+}
+
+// drifting-frame offsets of instance i after the steps run so far
+extern "C" void Formura_Batch_Offsets(FormuraBatch* b, int i, int* off) {
+  const Inst& I = b->inst[i];
+  const long long sh = (long long)SLEEVE * b->steps_done[i];
+  off[0] = (int)(((-sh) %% I.n1 + I.n1) %% I.n1);
+  off[1] = (int)(((-sh) %% I.n2 + I.n2) %% I.n2);
+}
+
+// nsteps time steps of every instance; returns the device time in ms (HIP events
+// on the batch's stream).  Full K-step launches are replayed from a hipGraph of
+// GRAPH_LAUNCHES launches.
+extern "C" double Formura_Batch_Step(FormuraBatch* b, int nsteps) {
+  if (nsteps < 0) fmr_rt::fatal("Formura_Batch_Step: negative count");
+  b->launches = 0;
+  FMR_CHECK(hipEventRecord(b->ev0, b->stream));
+  long long full = nsteps / KMAX;
+  const int rem = nsteps %% KMAX;
+  if (full >= GRAPH_LAUNCHES) {
+    const int par = (b->cur == b->buf[0]) ? 0 : 1;
+    if (!b->exec[par]) {
+      FMR_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      for (int g = 0; g < GRAPH_LAUNCHES; ++g) {
+        launch_k(b, KMAX, b->cur, b->nxt);
+        real_t* t = b->cur; b->cur = b->nxt; b->nxt = t;
+      }
+      FMR_CHECK(hipStreamEndCapture(b->stream, &b->graph[par]));
+      FMR_CHECK(hipGraphInstantiate(&b->exec[par], b->graph[par], nullptr, nullptr, 0));
+      b->launches -= GRAPH_LAUNCHES;
+    }
+    // graph `par` reads buf[par] and, after an even number of launches, leaves the
+    // state in buf[par] == cur again
+    while (full >= GRAPH_LAUNCHES) {
+      FMR_CHECK(hipGraphLaunch(b->exec[par], b->stream));
+      b->launches += GRAPH_LAUNCHES;
+      full -= GRAPH_LAUNCHES;
+    }
+  }
+  for (long long l = 0; l < full; ++l) {
+    launch_k(b, KMAX, b->cur, b->nxt);
+    real_t* t = b->cur; b->cur = b->nxt; b->nxt = t;
+  }
+  if (rem) {
+    launch_k(b, rem, b->cur, b->nxt);
+    real_t* t = b->cur; b->cur = b->nxt; b->nxt = t;
+  }
+  FMR_CHECK(hipGetLastError());
+  FMR_CHECK(hipEventRecord(b->ev1, b->stream));
+  FMR_CHECK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  FMR_CHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  for (int i = 0; i < b->ninst; ++i) b->steps_done[i] += nsteps;
+  return ms;
+}
+
+extern "C" long long Formura_Batch_Last_Launches(FormuraBatch* b) { return b->launches; }
+extern "C" long long Formura_Batch_Cells(FormuraBatch* b) { return b->total / NSLOT; }
+
+extern "C" void Formura_Batch_Destroy(FormuraBatch* b) {
+  if (!b) return;
+  FMR_CHECK(hipStreamSynchronize(b->stream));
+  for (int p = 0; p < 2; ++p) {
+    if (b->exec[p]) FMR_CHECK(hipGraphExecDestroy(b->exec[p]));
+    if (b->graph[p]) FMR_CHECK(hipGraphDestroy(b->graph[p]));
+  }
+  fmr_rt::device_free(b->inst_d);
+  fmr_rt::device_free(b->tile_inst_d);
+  fmr_rt::device_free(b->cur);
+  fmr_rt::device_free(b->nxt);
+  FMR_CHECK(hipEventDestroy(b->ev0));
+  FMR_CHECK(hipEventDestroy(b->ev1));
+  FMR_CHECK(hipStreamDestroy(b->stream));
+  delete b;
+}
+'''

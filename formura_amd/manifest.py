@@ -31,6 +31,7 @@ class Micro:
                      # ("LoadIndex", axis) ("Uniop", op, a) ("Binop", op, a, b) ("Triop", op, a, b, c)
                      # ("Store", name, a)
     typ: str         # element type name (micro types are element types only)
+    src: str = None  # source name of the OM node (set for literals bound to a name)
 
 
 @dataclasses.dataclass
@@ -67,7 +68,8 @@ def manifest_graph(g: Graph, dim: int) -> Dict[int, MMNode]:
             if key not in idmap:
                 idmap[key] = len(idmap)
             j = idmap[key]
-            micro[j] = Micro(inst, _micro_type(g.nodes[i].typ))
+            micro[j] = Micro(inst, _micro_type(g.nodes[i].typ),
+                             g.nodes[i].name if inst[0] not in ("LoadCursorStatic", "LoadCursor", "Store") else None)
             return j
 
         def code_at(c, nid):
@@ -206,12 +208,16 @@ def inline_temporaries(mm: Dict[int, MMNode]) -> Dict[int, MMNode]:
                 r = copy(tgt, _add(m.inst[1], shift), len(tgt.micro) - 1, {})
             elif k0 == "Imm" or k0 == "LoadIndex":
                 r = emit(m.inst, m.typ)
+                if k0 == "Imm" and m.src:
+                    new[r].src = m.src
             elif k0 == "Uniop":
                 r = emit(("Uniop", m.inst[1], copy(src, shift, m.inst[2], cache)), m.typ)
+                new[r].src = new[r].src or m.src
             elif k0 == "Binop":
                 a = copy(src, shift, m.inst[2], cache)
                 b = copy(src, shift, m.inst[3], cache)
                 r = emit(("Binop", m.inst[1], a, b), m.typ)
+                new[r].src = new[r].src or m.src
             elif k0 == "Triop":
                 a = copy(src, shift, m.inst[2], cache)
                 b = copy(src, shift, m.inst[3], cache)

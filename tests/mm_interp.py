@@ -19,13 +19,14 @@ def _shift(a, offs):
     return np.roll(a, shift=[-o for o in offs], axis=tuple(range(a.ndim)))
 
 
-def _eval_node(node, fields, temps, base):
+def _eval_node(node, fields, temps, base, params=None):
     vals = []
+    params = params or {}
     ext = {"sqrt": np.sqrt, "exp": np.exp, "fabs": np.fabs}
     for m in node.micro:
         k = m.inst[0]
         if k == "Imm":
-            v = float(m.inst[1])
+            v = float(params[m.src]) if m.src in params else float(m.inst[1])
         elif k == "LoadCursorStatic":
             v = _shift(fields[m.inst[2]], [c + b for c, b in zip(m.inst[1], base)])
         elif k == "LoadCursor":
@@ -63,7 +64,7 @@ def _eval_node(node, fields, temps, base):
     return vals[-1]
 
 
-def step(mm_graph, sleeve, fields):
+def step(mm_graph, sleeve, fields, params=None):
     """One application of a step graph to a dict of periodic numpy fields."""
     rng = calc_range(mm_graph)
     shape = next(iter(fields.values())).shape
@@ -72,7 +73,7 @@ def step(mm_graph, sleeve, fields):
         node = mm_graph[k]
         is_store = node.micro[-1].inst[0] == "Store"
         base = [rng[k].offset - 2 * sleeve] * len(shape) if is_store else [0] * len(shape)
-        v = np.broadcast_to(np.asarray(_eval_node(node, fields, temps, base), dtype=np.float64), shape)
+        v = np.broadcast_to(np.asarray(_eval_node(node, fields, temps, base, params), dtype=np.float64), shape)
         if is_store:
             out[node.micro[-1].inst[1]] = np.array(v)
         else:

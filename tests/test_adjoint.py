@@ -1,0 +1,165 @@
+"""Adjoint (reverse-mode) kernels behind torch.autograd, with binomial checkpointing.
+
+* the adjoint library's forward at the program's own parameters is bit-exact
+  with the oracle;
+* gradients w.r.t. the initial fields and the parameters F, k, Du, Dv match an
+  independent FP64 torch CPU autograd reference of the Gray-Scott step (and of
+  the heat step) to 1e-10;
+* gradients match FP64 central differences (eps = 1e-6) along seeded random
+  directions to 1e-6;
+* the snapshot count changes the schedule, not the gradient (bit-identical).
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import layout_of, oracle_lib
+from formura_amd import adjoint as A
+from formura_amd import batch as B
+from formura_amd import build, inputs
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+THETA = [0.04, 0.06, 2e-5, 1e-5]
+
+
+def _stepper(name):
+    import os
+    if not os.path.exists(build.lib_path(name)):
+        build.build_adjoint(name)
+    return A.Stepper(name)
+
+
+def torch_gs_step(u, v, F, k, Du, Dv, dt=1.0, dx=0.01):
+    """Gray-Scott step in the drifting frame: output i holds position i-1."""
+    c = lambda x: torch.roll(x, (1, 1), (0, 1))
+    xp = lambda x: torch.roll(x, (0, 1), (0, 1))
+    xm = lambda x: torch.roll(x, (2, 1), (0, 1))
+    yp = lambda x: torch.roll(x, (1, 0), (0, 1))
+    ym = lambda x: torch.roll(x, (1, 2), (0, 1))
+    lap = lambda x: ((xp(x) + xm(x)) - 2 * c(x)) / (dx * dx) + ((yp(x) + ym(x)) - 2 * c(x)) / (dx * dx)
+    uc, vc = c(u), c(v)
+    return (uc + dt * ((Du * lap(u) - uc * vc * vc) + F * (1 - uc)),
+            vc + dt * ((Dv * lap(v) + uc * vc * vc) - (F + k) * vc))
+
+
+def _gs_problem(n=64, steps=30, seed=3):
+    st = B.grayscott_init(n, seed)
+    x0 = np.stack([st["u"], st["v"]])
+    s = _stepper("gs_%d_adj" % n)
+    tgt = A.run(s, torch.tensor(x0, device=DEV), torch.tensor([0.035, 0.065, 2e-5, 1e-5], dtype=torch.float64), steps, 2)
+    return s, x0, tgt[1].detach()
+
+
+def _gpu_loss_and_grads(s, x0, theta, vt, steps, snaps):
+    x = torch.tensor(x0, device=DEV, requires_grad=True)
+    p = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    out = A.run(s, x, p, steps, snaps)
+    loss = 0.5 * ((out[1] - vt) ** 2).sum()
+    loss.backward()
+    return loss.item(), x.grad.cpu().numpy(), p.grad.cpu().numpy()
+
+
+def test_adjoint_library_forward_is_bit_exact():
+    s = _stepper("gs_64_adj")
+    o = oracle_lib("gs_64", layout_of("gs_64"))
+    o.init()
+    st = B.grayscott_init(64, 9)
+    o.set_state(st)
+    o.forward(57)
+    x = torch.tensor(np.stack([st["u"], st["v"]]), device=DEV)
+    out = A.run(s, x, torch.tensor(THETA, dtype=torch.float64), 57, 0).cpu().numpy()
+    np.testing.assert_array_equal(out[0], o.field("u"))
+    np.testing.assert_array_equal(out[1], o.field("v"))
+
+
+def test_gradients_match_torch_cpu_reference():
+    steps = 30
+    s, x0, vt = _gs_problem(steps=steps)
+    loss, gx, gp = _gpu_loss_and_grads(s, x0, THETA, vt, steps, 3)
+    u = torch.tensor(x0[0], requires_grad=True)
+    v = torch.tensor(x0[1], requires_grad=True)
+    th = torch.tensor(THETA, dtype=torch.float64, requires_grad=True)
+    a, b = u, v
+    for _ in range(steps):
+        a, b = torch_gs_step(a, b, th[0], th[1], th[2], th[3])
+    ref = 0.5 * ((b - vt.cpu()) ** 2).sum()
+    ref.backward()
+    assert abs(loss - ref.item()) <= 1e-12 * abs(ref.item())
+    for got, want in ((gx[0], u.grad.numpy()), (gx[1], v.grad.numpy()), (gp, th.grad.numpy())):
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        assert err < 1e-10, err
+
+
+def test_gradients_match_central_differences():
+    steps = 30
+    s, x0, vt = _gs_problem(steps=steps)
+    loss, gx, gp = _gpu_loss_and_grads(s, x0, THETA, vt, steps, 4)
+    rng = np.random.default_rng(11)
+    scale = np.array(THETA)
+    eps = 1e-6
+
+    def L(x, th):
+        with torch.no_grad():
+            out = A.run(s, torch.tensor(x, device=DEV), torch.tensor(th, dtype=torch.float64), steps, 0)
+            return 0.5 * float(((out[1] - vt) ** 2).sum())
+
+    for it in range(8):
+        dx = rng.standard_normal(x0.shape)
+        dp = rng.standard_normal(4) * scale
+        fd = (L(x0 + eps * dx, np.array(THETA) + eps * dp) - L(x0 - eps * dx, np.array(THETA) - eps * dp)) / (2 * eps)
+        an = float((gx * dx).sum() + (gp * dp).sum())
+        assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-30), (it, fd, an)
+
+
+def test_snapshot_count_does_not_change_gradients():
+    steps = 30
+    s, x0, vt = _gs_problem(steps=steps)
+    res = [_gpu_loss_and_grads(s, x0, THETA, vt, steps, snaps) for snaps in (0, 2, 32)]
+    for r in res[1:]:
+        np.testing.assert_array_equal(r[1], res[0][1])
+        np.testing.assert_array_equal(r[2], res[0][2])
+    assert s.last_stats["forward_steps"] == A.forward_steps_needed(steps, 32)
+
+
+def test_heat_adjoint_matches_torch_cpu_reference():
+    s = _stepper("heat3d_16_adj")
+    n, steps = 16, 12
+    q0 = inputs.uniform(5, (n, n, n))
+    tgt = torch.tensor(inputs.uniform(6, (n, n, n)))
+    x = torch.tensor(q0[None], device=DEV, requires_grad=True)
+    out = A.run(s, x, torch.zeros(0, dtype=torch.float64), steps, 2)
+    loss = 0.5 * ((out[0] - tgt.to(DEV)) ** 2).sum()
+    loss.backward()
+    q = torch.tensor(q0, requires_grad=True)
+    a = q
+    dx = 1.0 / n
+    for _ in range(steps):
+        c = torch.roll(a, (1, 1, 1), (0, 1, 2))
+        lap = 0
+        for ax in range(3):
+            sp = [1, 1, 1]
+            sm = [1, 1, 1]
+            sp[ax] = 0
+            sm[ax] = 2
+            lap = lap + ((torch.roll(a, sp, (0, 1, 2)) + torch.roll(a, sm, (0, 1, 2))) - 2 * c) / dx ** 2
+        a = c + (dx * dx / 8) * lap
+    ref = 0.5 * ((a - tgt) ** 2).sum()
+    ref.backward()
+    g = x.grad[0].cpu().numpy()
+    assert np.linalg.norm(g - q.grad.numpy()) / np.linalg.norm(q.grad.numpy()) < 1e-12
+
+
+def test_fp32_adjoint_close_to_fp64():
+    steps = 30
+    s, x0, vt = _gs_problem(steps=steps)
+    _, gx, gp = _gpu_loss_and_grads(s, x0, THETA, vt, steps, 3)
+    s32 = _stepper("gs_64_adj_f32")
+    x = torch.tensor(x0, device=DEV, dtype=torch.float32, requires_grad=True)
+    p = torch.tensor(THETA, dtype=torch.float64, requires_grad=True)
+    out = A.run(s32, x, p, steps, 3)
+    loss = 0.5 * ((out[1] - vt.float()) ** 2).sum()
+    loss.backward()
+    g32 = x.grad.double().cpu().numpy()
+    err = np.linalg.norm(g32 - gx) / np.linalg.norm(gx)
+    assert err < 1e-3, err

@@ -1,0 +1,75 @@
+"""Binomial checkpointing (CPU): the schedule is optimal and reverses correctly."""
+import functools
+
+import pytest
+import torch
+
+import formura_amd.adjoint as A
+
+
+@functools.lru_cache(maxsize=None)
+def brute(l, s):
+    """Exhaustive minimum of forward steps: advance j, reverse the tail with s-1, the head with s."""
+    if l <= 1:
+        return 0
+    if s == 0:
+        return l * (l - 1) // 2
+    return min(j + brute(l - j, s - 1) + brute(j, s) for j in range(1, l))
+
+
+@pytest.mark.parametrize("s", [0, 1, 2, 3, 5])
+def test_schedule_is_optimal(s):
+    for l in range(1, 60):
+        assert A.forward_steps_needed(l, s) == brute(l, s) == A.optimal_cost(l, s), (l, s)
+
+
+def test_config_counts():
+    # 2000 steps, 32 free snapshots (33 checkpoints with the initial state):
+    # repetition number 3, 3*2000 - C(36, 2) recomputed steps
+    assert A.repetitions(2000, 32) == 3
+    assert A.forward_steps_needed(2000, 32) == 3 * 2000 - 630
+
+
+class MockStepper:
+    """x -> M x with a fixed random matrix; adjoint: lam -> M^T lam.  CPU only."""
+
+    def __init__(self, n=6, seed=0):
+        g = torch.Generator().manual_seed(seed)
+        self.M = torch.randn(n, n, dtype=torch.float64, generator=g) / n ** 0.5
+        self.dtype = torch.float64
+        self.nparams = 0
+        self.slots = 1
+        self.calls = []
+
+    def forward(self, x, out, tmp, nsteps, p):
+        y = x.clone()
+        for _ in range(nsteps):
+            y = self.M @ y
+        out.copy_(y)
+        self.calls.append(("F", nsteps))
+
+    def adjoint(self, state, lam, lam_out, p, pgrad):
+        lam_out.copy_(self.M.T @ lam)
+        self.calls.append(("A", state.clone()))
+
+
+@pytest.mark.parametrize("steps,snaps", [(1, 3), (7, 0), (13, 2), (40, 3), (25, 32)])
+def test_reverse_schedule_with_mock_stepper(steps, snaps):
+    st = MockStepper()
+    x0 = torch.randn(6, dtype=torch.float64, requires_grad=True)
+    y = A._Run.apply(x0, torch.zeros(0, dtype=torch.float64), st, steps, snaps)
+    w = torch.randn(6, dtype=torch.float64)
+    (y * w).sum().backward()
+    expect = w.clone()
+    for _ in range(steps):
+        expect = st.M.T @ expect
+    assert torch.allclose(x0.grad, expect, rtol=1e-12, atol=1e-14)
+    # the adjoint saw the states x_{T-1}, ..., x_0 in this order
+    seen = [c[1] for c in st.calls if c[0] == "A"]
+    states = [x0.detach().clone()]
+    for _ in range(steps - 1):
+        states.append(st.M @ states[-1])
+    assert len(seen) == steps
+    for a, b in zip(seen, reversed(states)):
+        assert torch.allclose(a, b, rtol=1e-13, atol=1e-15)
+    assert st.last_stats["forward_steps"] == A.forward_steps_needed(steps, snaps)
