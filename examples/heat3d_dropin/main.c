@@ -1,5 +1,5 @@
 /* A user driver written against the generated C API only (the way the
- * reference's examples/*/main.c are): Formura_Init, a Formura_Forward loop,
+ * reference's example drivers are): Formura_Init, a Formura_Forward loop,
  * direct reads of formura_data and to_pos_*, Formura_Finalize.  It links
  * unchanged against the reference's generated C or against the HIP library.
  * Prints a checksum of the final state and the Navi offsets. */

@@ -3,15 +3,27 @@ adjoint steps (``include/formura_adjoint.h``), with binomial checkpointing.
 
 ``Stepper`` binds one adjoint library.  ``run(state0, params, nsteps,
 snapshots)`` is a ``torch.autograd.Function``.  Its forward advances
-``nsteps`` steps and keeps only ``snapshots`` states in HBM.  Its backward
+``nsteps`` steps and keeps at most ``snapshots`` states in HBM: the
+checkpoints of the schedule's first path.  Its backward
 reverses the run with the binomial checkpointing schedule of Griewank and
 Walther ("Algorithm 799: Revolve", ACM TOMS 26(1), 2000).  A segment of l
 steps with s free snapshots (c = s+1 checkpoints counting the segment's first
 state) and repetition number r (the least r with C(c+r, c) >= l) is split after
 j steps, j chosen in the interval where both parts stay within r repetitions;
 the tail is reversed with s-1 snapshots and the head with s.  Forward steps
-recomputed: r*l - C(c+r, r-1), the binomial optimum.  Each reversed step calls the adjoint kernel at the
-recomputed input state.
+of the whole schedule: r*l - C(c+r, r-1), the binomial optimum; the forward
+sweep does the first path's share of them.
+
+The schedule runs over units of ``Formura_AD_Levels()`` steps (4 for the 2-D
+tiled kernels): checkpoints sit at unit boundaries, and one fused call
+(``Formura_AD_Adjoint_Steps``) reverses a unit from the state at its start,
+recomputing the unit's inner states into a scratch of levels-1 states.  For
+2000 steps and 32 snapshots this turns 2000 steps at repetition number 3 into
+500 units at repetition number 2.
+
+A step that is linear in the state and has no parameters (the heat equation)
+has an adjoint that reads no state: its reversal runs the adjoint steps alone,
+with no checkpoints and no recomputation.
 
 The result is dL/d(initial state) and dL/d(parameters).  The parameters are
 the program's named literals listed by the library (for Gray-Scott: F, k, Du,
@@ -75,20 +87,65 @@ def split(l: int, s: int) -> int:
     return _SPLIT[key]
 
 
-def forward_steps_needed(l: int, s: int) -> int:
-    """Forward steps the reversal recomputes (the first forward sweep excluded),
-    following exactly the schedule of ``_Run.backward``."""
+def first_descent(l: int, s: int) -> List[int]:
+    """Step counts between the checkpoints that the forward sweep keeps: the
+    schedule's first path (advance split(l, s), keep the state, continue with
+    s-1 snapshots) until one step or no snapshot is left."""
+    js = []
+    while l > 1 and s > 0:
+        j = split(l, s)
+        js.append(j)
+        l -= j
+        s -= 1
+    return js
+
+
+def forward_steps_needed(l: int, s: int, stored: bool = True) -> int:
+    """Forward steps the reversal recomputes, following exactly the schedule of
+    ``_Run.backward``.  With ``stored`` the forward sweep has kept the
+    checkpoints of ``first_descent``, so those advances are not repeated."""
     total = 0
-    stack = [(l, s)]
+    stack = [(l, s, stored)]
     while stack:
-        l, s = stack.pop()
+        l, s, pre = stack.pop()
         while l > 1:
             if s == 0:
                 total += l * (l - 1) // 2
                 break
             j = split(l, s)
-            total += j
-            stack.append((l - j, s - 1))
+            if not pre:
+                total += j
+            stack.append((l - j, s - 1, pre))
+            pre = False
+            l = j
+    return total
+
+
+def unit_steps(nsteps: int, unit: int, a: int, b: int) -> int:
+    """Steps from unit a to unit b when ``nsteps`` steps are cut into units of
+    ``unit`` steps (the last unit may be shorter)."""
+    return min(nsteps, b * unit) - min(nsteps, a * unit)
+
+
+def recompute_steps(nsteps: int, s: int, unit: int = 1) -> int:
+    """Forward steps the reversal of ``_Run.backward`` recomputes (the forward
+    sweep's kept checkpoints excluded) when the schedule runs over units of
+    ``unit`` steps; the inner states of a unit are recomputed inside the fused
+    adjoint call and are not counted here."""
+    l = -(-nsteps // unit) if nsteps > 0 else 0
+    total = 0
+    stack = [(0, l, s, True)]
+    while stack:
+        pos, l, s, pre = stack.pop()
+        while l > 1:
+            if s == 0:
+                total += sum(unit_steps(nsteps, unit, pos, pos + j) for j in range(1, l))
+                break
+            j = split(l, s)
+            if not pre:
+                total += unit_steps(nsteps, unit, pos, pos + j)
+            stack.append((pos + j, l - j, s - 1, pre))
+            pre = False
             l = j
     return total
 
@@ -105,12 +162,18 @@ class Stepper:
         L.Formura_AD_Describe.restype = ctypes.c_char_p
         self.desc = json.loads(L.Formura_AD_Describe().decode())
         L.Formura_AD_Param_Slots.restype = ctypes.c_int
+        L.Formura_AD_Levels.restype = ctypes.c_int
         L.Formura_AD_Forward.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.Formura_AD_Adjoint.argtypes = [ctypes.c_void_p] * 6
+        L.Formura_AD_Adjoint_Steps.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int] + [ctypes.c_void_p] * 3
         self.dtype = torch.float64 if self.desc["real"] == "double" else torch.float32
         self.shape = [len(self.desc["fields"])] + list(self.desc["grid"])
         self.nparams = len(self.desc["params"])
         self.slots = int(L.Formura_AD_Param_Slots())
+        # most steps one fused adjoint call reverses
+        self.levels = int(L.Formura_AD_Levels())
+        # a step linear in the state with no parameters: its adjoint reads no state
+        self.state_free = bool(self.desc.get("state_free"))
 
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -118,6 +181,10 @@ class Stepper:
     def _check(self, t):
         if not (t.is_cuda and t.dtype == self.dtype and list(t.shape) == self.shape and t.is_contiguous()):
             raise ValueError("expected a contiguous %s CUDA tensor of shape %s" % (self.dtype, self.shape))
+
+    def scratch(self, device):
+        """Inner-state buffer of the fused adjoint (levels - 1 states)."""
+        return torch.empty([max(1, self.levels - 1)] + self.shape, dtype=self.dtype, device=device)
 
     def forward(self, x, out, tmp, nsteps, params_dev):
         self._check(x)
@@ -132,18 +199,45 @@ class Stepper:
         self.lib.Formura_AD_Adjoint(state_prev.data_ptr(), lam_next.data_ptr(), lam_prev.data_ptr(),
                                     params_dev.data_ptr(), pgrad.data_ptr(), self._stream())
 
+    def adjoint_steps(self, state0, scratch, lam_next, lam_prev, nsteps, params_dev, pgrad):
+        for t in (state0, lam_next, lam_prev):
+            self._check(t)
+        if not 1 <= nsteps <= self.levels:
+            raise ValueError("adjoint_steps reverses 1..%d steps, got %d" % (self.levels, nsteps))
+        if scratch.dtype != self.dtype or scratch.numel() < (nsteps - 1) * state0.numel() or not scratch.is_contiguous():
+            raise ValueError("scratch holds fewer than %d states" % (nsteps - 1))
+        self.lib.Formura_AD_Adjoint_Steps(state0.data_ptr(), scratch.data_ptr(), lam_next.data_ptr(),
+                                          lam_prev.data_ptr(), int(nsteps), params_dev.data_ptr(),
+                                          pgrad.data_ptr(), self._stream())
+
 
 class _Run(torch.autograd.Function):
+    """The schedule runs over units of ``stepper.levels`` steps (the fused
+    adjoint reverses one unit per call): checkpoints sit at unit boundaries and
+    ``snapshots`` counts them."""
+
     @staticmethod
     def forward(ctx, state0, params, stepper, nsteps, snapshots):
         st = stepper
+        n, unit = int(nsteps), max(1, getattr(st, "levels", 1))
         p = params.detach().to(device=state0.device, dtype=torch.float64).contiguous()
         x0 = state0.detach().contiguous()
-        out = torch.empty_like(x0)
         tmp = torch.empty_like(x0)
-        st.forward(x0, out, tmp, nsteps, p)
+        # the sweep keeps the checkpoints of the schedule's first path, so the
+        # reversal does not recompute them
+        kept, base, pos = [], x0, 0
+        nunits = -(-n // unit)
+        path = [] if getattr(st, "state_free", False) else first_descent(nunits, int(snapshots))
+        for j in path:
+            nxt = torch.empty_like(x0)
+            st.forward(base, nxt, tmp, unit_steps(n, unit, pos, pos + j), p)
+            kept.append(nxt)
+            base, pos = nxt, pos + j
+        out = torch.empty_like(x0)
+        st.forward(base, out, tmp, n - unit_steps(n, unit, 0, pos), p)
         ctx.save_for_backward(x0, p)
-        ctx.stepper, ctx.nsteps, ctx.snapshots = st, int(nsteps), int(snapshots)
+        ctx.kept = kept
+        ctx.stepper, ctx.nsteps, ctx.snapshots = st, n, int(snapshots)
         ctx.param_device, ctx.param_dtype = params.device, params.dtype
         return out
 
@@ -151,44 +245,64 @@ class _Run(torch.autograd.Function):
     def backward(ctx, grad_out):
         x0, p = ctx.saved_tensors
         st = ctx.stepper
+        n, unit = ctx.nsteps, max(1, getattr(st, "levels", 1))
         lam = grad_out.detach().to(st.dtype).contiguous().clone()
         lam2 = torch.empty_like(lam)
         pgrad = torch.zeros(max(1, st.slots), max(1, st.nparams), dtype=torch.float64, device=x0.device)
         tmp = torch.empty_like(x0)
-        stats = {"forward_steps": 0, "adjoint_steps": 0}
+        scratch = st.scratch(x0.device) if unit > 1 else None
+        stats = {"forward_steps": 0, "adjoint_steps": 0, "inner_forward_steps": 0, "adjoint_calls": 0}
 
-        def advance(src, nsteps):
+        def advance(src, pos, j):
+            k = unit_steps(n, unit, pos, pos + j)
             dst = torch.empty_like(src)
-            st.forward(src, dst, tmp, nsteps, p)
-            stats["forward_steps"] += nsteps
+            st.forward(src, dst, tmp, k, p)
+            stats["forward_steps"] += k
             return dst
 
-        def adjoint_at(state):
+        def adjoint_unit(state, pos):
             nonlocal lam, lam2
-            st.adjoint(state, lam, lam2, p, pgrad)
+            k = unit_steps(n, unit, pos, pos + 1)
+            if unit == 1:
+                st.adjoint(state, lam, lam2, p, pgrad)
+            else:
+                st.adjoint_steps(state, scratch, lam, lam2, k, p, pgrad)
             lam, lam2 = lam2, lam
-            stats["adjoint_steps"] += 1
+            stats["adjoint_steps"] += k
+            stats["inner_forward_steps"] += k - 1
+            stats["adjoint_calls"] += 1
 
-        def reverse(base, l, s):
-            # base: state at the start of the segment; reverse its l steps with s free
-            # snapshots.  The tail recursion over the head of the segment is a loop,
-            # so the depth is at most s.
+        def reverse(base, pos, l, s, kept):
+            # base: state at unit `pos`, the start of the segment; reverse its l
+            # units with s free snapshots.  `kept` holds the states the forward
+            # sweep kept along this segment's first path.  The tail recursion over
+            # the head of the segment is a loop, so the depth is at most s.
             while True:
                 if l == 1:
-                    adjoint_at(base)
+                    adjoint_unit(base, pos)
                     return
                 if s == 0:
                     for j in range(l - 1, -1, -1):
-                        adjoint_at(advance(base, j) if j else base)
+                        adjoint_unit(advance(base, pos, j) if j else base, pos + j)
                     return
                 j = split(l, s)
-                mid = advance(base, j)
-                reverse(mid, l - j, s - 1)
+                mid = kept.pop(0) if kept else advance(base, pos, j)
+                reverse(mid, pos + j, l - j, s - 1, kept)
+                kept = []
                 del mid
                 l = j
 
-        if ctx.nsteps > 0:
-            reverse(x0, ctx.nsteps, ctx.snapshots)
+        if getattr(st, "state_free", False):
+            # the adjoint of a state-independent linear step needs no forward state
+            for _ in range(n):
+                st.adjoint(x0, lam, lam2, p, pgrad)
+                lam, lam2 = lam2, lam
+                stats["adjoint_steps"] += 1
+                stats["adjoint_calls"] += 1
+        elif n > 0:
+            kept = list(ctx.kept)
+            ctx.kept = None
+            reverse(x0, 0, -(-n // unit), ctx.snapshots, kept)
         ctx.stepper.last_stats = stats
         gp = pgrad.sum(0)[: st.nparams].to(device=ctx.param_device, dtype=ctx.param_dtype) if st.nparams else None
         return lam.to(grad_out.dtype), (gp if gp is not None else None), None, None, None

@@ -40,6 +40,7 @@ PROGRAMS: Dict[str, tuple] = {
     "heat3d_256": ("heat3d.fmr", "heat3d_256.yaml", "auto"),
     "heat3d_512": ("heat3d.fmr", "heat3d_512.yaml", "auto"),
     "gs_64": ("grayscott2d.fmr", "gs_64.yaml", "auto"),
+    "gs_1024": ("grayscott2d.fmr", "gs_1024.yaml", "auto"),
     "gs_2048": ("grayscott2d.fmr", "gs_2048.yaml", "auto"),
     "gs_8192": ("grayscott2d.fmr", "gs_8192.yaml", "auto"),
     "euler3d_16": ("euler3d.fmr", "euler3d_16.yaml", "auto"),

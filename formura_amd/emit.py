@@ -184,11 +184,13 @@ class Body:
 
     def __init__(self, node: MMNode, load: Callable, tmp_load: Callable, index: Callable,
                  store: Callable, prefix: str, fold: bool = True, const_kw="const ",
-                 params: Dict[str, str] = None, real: str = None):
+                 params: Dict[str, str] = None, real: str = None, recip: bool = False):
         """``params``: literal names bound to run-time expressions (not folded);
         ``real``: if set ("float"), every floating micro value is computed in
         that type (the FP32 form; constants are folded in double and rounded,
-        and a division by a constant becomes a multiplication by its reciprocal)."""
+        and a division by a constant becomes a multiplication by its reciprocal);
+        ``recip``: divide by constants through their reciprocal in double too
+        (not bit-exact: for derivative code, never for a step's own values)."""
         self.lines: List[str] = []
         vals: Dict[int, object] = {}
         params = params or {}
@@ -217,10 +219,10 @@ class Body:
                 self.lines.append("%s%s %s = (%s%d - %s%d) + (%s%d - %s%d);"
                                   % (const_kw, ct, name, prefix, pa, prefix, pc, prefix, pb, prefix, pc))
                 continue
-            if real and k == "Binop" and m.inst[1] == "/" and m.inst[3] in vals and m.inst[2] not in vals:
+            if (real or recip) and k == "Binop" and m.inst[1] == "/" and m.inst[3] in vals and m.inst[2] not in vals:
                 rc = 1.0 / vals[m.inst[3]] if vals[m.inst[3]] else float("inf")
-                self.lines.append("%s%s %s = %s%d*%s;" % (const_kw, ct, name, prefix, m.inst[2],
-                                                        _const_literal(_f32(rc), "float")))
+                lit = _const_literal(_f32(rc), "float") if real == "float" else show_double(rc)
+                self.lines.append("%s%s %s = %s%d*%s;" % (const_kw, ct, name, prefix, m.inst[2], lit))
                 continue
             if fold:
                 ops = _operands(m)
@@ -376,3 +378,33 @@ def lib_description(om, name: str, schedule: str) -> dict:
         "filter_interval": cfg.filter_interval,
         "schedule": schedule,
     }
+
+
+def tile_load(count: str, index_lines: List[str], pairs: List[tuple], ctype: str, tag: str = "ld",
+              threads: int = 256, indent: str = "  ", max_count: Optional[str] = None) -> str:
+    """C statements loading a tile of ``count`` elements (a constant expression)
+    into LDS with ``threads`` work-items: every global load of the work-item is
+    issued (unrolled, into registers) before the first LDS write, so the tile
+    costs one memory latency, not one per loop trip.  ``index_lines`` compute
+    ``g`` from ``idx``; ``pairs`` are (lds_element, global_element) in terms of
+    ``idx`` and ``g``; all are of C type ``ctype``.  ``max_count`` (a constant
+    expression) bounds ``count`` when ``count`` is not itself constant."""
+    it = "%s_IT" % tag
+    out = ["%sconstexpr int %s = (%s + %d) / %d;" % (indent, it, max_count or count, threads - 1, threads)]
+    out.append("%s%s %s;" % (indent, ctype, ", ".join("%s_v%d[%s]" % (tag, i, it) for i in range(len(pairs)))))
+    out.append("%s#pragma unroll" % indent)
+    out.append("%sfor (int it = 0; it < %s; ++it) {" % (indent, it))
+    out.append("%s  const int idx = threadIdx.x + it * %d;" % (indent, threads))
+    out.append("%s  if (idx < (%s)) {" % (indent, count))
+    out += ["%s    %s" % (indent, l) for l in index_lines]
+    out += ["%s    %s_v%d[it] = %s;" % (indent, tag, i, src) for i, (dst, src) in enumerate(pairs)]
+    out.append("%s  }" % indent)
+    out.append("%s}" % indent)
+    out.append("%s#pragma unroll" % indent)
+    out.append("%sfor (int it = 0; it < %s; ++it) {" % (indent, it))
+    out.append("%s  const int idx = threadIdx.x + it * %d;" % (indent, threads))
+    out.append("%s  if (idx < (%s)) {" % (indent, count))
+    out += ["%s    %s = %s_v%d[it];" % (indent, dst, tag, i) for i, (dst, src) in enumerate(pairs)]
+    out.append("%s  }" % indent)
+    out.append("%s}" % indent)
+    return "\n".join(out)

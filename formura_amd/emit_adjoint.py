@@ -33,7 +33,7 @@ import math
 from typing import Dict, List
 
 from . import syntax as S
-from .emit import Body, EmitError, GraphPlan, _const_literal, _f32, show_double, state_fields
+from .emit import Body, EmitError, tile_load, GraphPlan, _const_literal, _f32, show_double, state_fields
 from .manifest import calc_range, inline_temporaries
 
 
@@ -147,9 +147,15 @@ class ReverseSweep:
 
 
 class AdjointEmitter:
-    def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256):
+    def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256,
+                 tile=(16, 32, 3), adj_tile=None, levels=4):
         self.mm, self.om, self.name = mm, mm.om, name
         self.real = real
+        self.tile = tuple(tile) if tile else None
+        # tile of the adjoint kernel (defaults to the forward kernel's)
+        self.adj_tile = tuple(adj_tile) if adj_tile else (self.tile[:2] if self.tile else None)
+        # most steps one fused adjoint launch reverses (2-D tiled kernels)
+        self.levels = max(1, int(levels))
         self.dim = self.om.dimension
         self.N = self.om.config.grid_per_node
         self.params = list(params)
@@ -197,49 +203,67 @@ class AdjointEmitter:
                 rem = "(%s / N%d)" % (rem, k + 1)
         return lines
 
-    def _forward_body(self, k, shift, prefix):
-        """Forward instructions of Store node k at output point (q + shift)."""
+    def _forward_body(self, k, shift, prefix, xload, exact=True):
+        """Forward instructions of Store node k at output point (q + shift); the
+        state value of field f at q + a is ``xload(f, a)``.  With ``exact`` off
+        (values that only feed derivatives) constant divisors become reciprocal
+        multiplications."""
         s = self.plan.sleeve
         base = self.plan.rng[k].offset - 2 * s
 
         def load(field, c):
-            return "FLD%d(st)[%s]" % (self.fi[field], self._idx(self._pos([shift[a] + c[a] + base for a in range(self.dim)])))
+            return xload(self.fi[field], [shift[a] + c[a] + base for a in range(self.dim)])
 
         pmap = {p: "P[%d]" % i for i, p in enumerate(self.params)}
-        b = Body(self.plan.mm[k], load, lambda *a: None, lambda ax: None, lambda n, v: "", prefix,
-                 params=pmap, real=("float" if self.real == "float" else None))
-        return b
+        return Body(self.plan.mm[k], load, lambda *a: None, lambda ax: None, lambda n, v: "", prefix,
+                    params=pmap, real=("float" if self.real == "float" else None), recip=not exact)
 
-    def emit(self) -> str:
+    def deltas(self):
+        """Distinct load offsets delta = c + base over all stores and fields."""
         s = self.plan.sleeve
-        dim = self.dim
-        R = self.real
-        nf = len(self.fields)
-        np_ = max(1, len(self.params))
-        # distinct load offsets delta = c + base, per (store, field)
-        deltas = []
+        out = []
         for k in self.plan.stores:
             base = self.plan.rng[k].offset - 2 * s
             for m in self.plan.mm[k].micro:
                 if m.inst[0] == "LoadCursorStatic":
                     d = tuple(c + base for c in m.inst[1])
-                    if d not in deltas:
-                        deltas.append(d)
-        deltas.sort()
-        body = self._decompose()
-        body += ["%s acc%d = 0;" % (R, g) for g in range(nf)]
-        for di, delta in enumerate(deltas):
+                    if d not in out:
+                        out.append(d)
+        return sorted(out)
+
+    @staticmethod
+    def _state_dependent(node):
+        on = []
+        for m in node.micro:
+            on.append(m.inst[0] in ("LoadCursorStatic", "LoadCursor", "LoadIndex")
+                      or any(on[o] for o in _operands(m.inst)))
+        return on
+
+    def point_body(self, xload, lload, pguard=None):
+        """Statements computing acc<g> (dL/d input field g at q) and pg<p> (the
+        parameter gradients of the output at q) for one point q; returns
+        (lines, uses_state): uses_state is False when no derivative reads a
+        state-dependent forward value (a step linear in the state with no
+        parameters: its adjoint does not need the forward states)."""
+        import re
+        s = self.plan.sleeve
+        dim = self.dim
+        R = self.real
+        nf = len(self.fields)
+        np_ = max(1, len(self.params))
+        uses_state = bool(self.params)
+        body = ["%s acc%d = 0;" % (R, g) for g in range(nf)]
+        for di, delta in enumerate(self.deltas()):
             shift = [-x for x in delta]  # output point i = q - delta
             body.append("{  // outputs that read this point at offset %s" % (delta,))
-            lam = ["LAM%d(lam_next)[%s]" % (self.fi[self.plan.mm[k].micro[-1].inst[1]], self._idx(self._pos(shift)))
-                   for k in self.plan.stores]
             for si, k in enumerate(self.plan.stores):
                 node = self.plan.mm[k]
                 base = self.plan.rng[k].offset - 2 * s
-                fb = self._forward_body(k, shift, "f%d_%d_" % (di, si))
+                fb = self._forward_body(k, shift, "f%d_%d_" % (di, si), xload, exact=False)
+                dep = self._state_dependent(node)
                 body += ["  " + l for l in fb.lines]
                 consts = getattr(fb, "vals", {})
-                body.append("  const %s lam%d_%d = %s;" % (R, di, si, lam[si]))
+                body.append("  const %s lam%d_%d = %s;" % (R, di, si, lload(self.fi[node.micro[-1].inst[1]], shift)))
                 for g, (gname, _) in enumerate(self.fields):
                     targets = {j for j, m in enumerate(node.micro)
                                if m.inst[0] == "LoadCursorStatic" and m.inst[2] == gname
@@ -250,62 +274,334 @@ class AdjointEmitter:
                     rs = ReverseSweep(node, targets, "lam%d_%d" % (di, si), "f%d_%d_" % (di, si),
                                       "b%d_%d_%d_" % (di, si, g), consts, R, leaves=pleaves)
                     if rs.result:
+                        for l in rs.lines:
+                            for j in re.findall(r"\bf%d_%d_(\d+)\b" % (di, si), l):
+                                if dep[int(j)] and int(j) not in consts:
+                                    uses_state = True
                         body += ["  " + l for l in rs.lines]
                         body.append("  acc%d += %s;" % (g, " + ".join(rs.result[j] for j in sorted(rs.result))))
             body.append("}")
         # parameter gradients at output i = q
-        pg = ["double pg%d = 0;" % p for p in range(np_)]
+        body += ["double pg%d = 0;" % p for p in range(np_)]
         if self.params:
-            pg.append("{")
+            body.append("if (%s) {" % pguard if pguard else "{")
             for si, k in enumerate(self.plan.stores):
                 node = self.plan.mm[k]
-                fb = self._forward_body(k, [0] * dim, "p%d_" % si)
-                pg += ["  " + l for l in fb.lines]
-                pg.append("  const %s plam%d = LAM%d(lam_next)[%s];"
-                          % (R, si, self.fi[node.micro[-1].inst[1]], self._idx(self._ivs())))
+                fb = self._forward_body(k, [0] * dim, "p%d_" % si, xload, exact=False)
+                body += ["  " + l for l in fb.lines]
+                body.append("  const %s plam%d = %s;" % (R, si, lload(self.fi[node.micro[-1].inst[1]], [0] * dim)))
                 for pi, p in enumerate(self.params):
                     leaves = {j for j, m in enumerate(node.micro) if m.src in self.params}
                     targets = {j for j, m in enumerate(node.micro) if m.src == p}
                     rs = ReverseSweep(node, targets, "plam%d" % si, "p%d_" % si, "pb%d_%d_" % (si, pi),
                                       getattr(fb, "vals", {}), R, leaves=leaves)
                     if rs.result:
-                        pg += ["  " + l for l in rs.lines]
-                        pg.append("  pg%d += (double)(%s);" % (pi, " + ".join(rs.result[j] for j in sorted(rs.result))))
-            pg.append("}")
-        body += pg
+                        body += ["  " + l for l in rs.lines]
+                        body.append("  pg%d += (double)(%s);" % (pi, " + ".join(rs.result[j] for j in sorted(rs.result))))
+            body.append("}")
+        return body, uses_state
+
+    def tiled(self):
+        """Tile shape for the LDS-tiled 2-D kernels, or None (3-D programs and
+        grids the tile does not divide use the one-point-per-thread kernels)."""
+        if self.dim != 2 or self.tile is None:
+            return None
+        tx, ty, k = self.tile
+        s = self.plan.sleeve
+        ax, ay = self.adj_tile
+        if (self.N[0] % tx or self.N[1] % ty or self.N[0] % ax or self.N[1] % ay
+                or 2 * (self.levels + 1) * s >= min(self.N) or 2 * k * s >= min(self.N)):
+            return None
+        return self.tile
+
+    def emit(self) -> str:
+        s = self.plan.sleeve
+        dim = self.dim
+        R = self.real
+        nf = len(self.fields)
+        np_ = max(1, len(self.params))
+        deltas = self.deltas()
+
+        def gidx(a):
+            return self._idx(self._pos(a))
+
+        flat, uses_state = self.point_body(lambda f, a: "FLD%d(st)[%s]" % (f, gidx(a)),
+                                           lambda f, a: "LAM%d(lam_next)[%s]" % (f, gidx(a)))
+        body = self._decompose() + flat
         body += ["if (live) LAM%d(lam_prev)[t] = acc%d;" % (g, g) for g in range(nf)]
-        # block reduction of the parameter gradients (FP64), one slot per block
-        body.append("__shared__ double red[%d][%d];" % (np_, self.block))
-        body += ["red[%d][threadIdx.x] = live ? pg%d : 0.0;" % (p, p) for p in range(np_)]
-        body.append("__syncthreads();")
-        body.append("for (int w = %d; w > 0; w >>= 1) {" % (self.block // 2))
-        body += ["  if ((int)threadIdx.x < w) red[%d][threadIdx.x] += red[%d][threadIdx.x + w];" % (p, p) for p in range(np_)]
-        body.append("  __syncthreads();")
-        body.append("}")
-        body.append("if (threadIdx.x == 0) {")
-        body += ["  pgrad[(long long)blockIdx.x * NP + %d] += red[%d][0];" % (p, p) for p in range(np_)]
-        body.append("}")
+        body += ["pg%d = live ? pg%d : 0.0;" % (p, p) for p in range(np_)]
+        body.append("{ const double pv[NP] = {%s}; block_reduce_params<%d>(pgrad + (long long)blockIdx.x * NP, pv); }"
+                    % (", ".join("pg%d" % p for p in range(np_)), self.block))
         # forward kernel (parameters at run time)
         fwd = self._decompose()
         fwd.append("if (!live) return;")
         for si, k in enumerate(self.plan.stores):
-            fb = self._forward_body(k, [0] * dim, "s%d_" % si)
+            fb = self._forward_body(k, [0] * dim, "s%d_" % si, lambda f, a: "FLD%d(st)[%s]" % (f, gidx(a)))
             fwd.append("{")
             fwd += ["  " + l for l in fb.lines]
             fwd.append("  FLD%d(out)[t] = s%d_%d;" % (self.fi[self.plan.mm[k].micro[-1].inst[1]], si,
                                                     self.plan.mm[k].micro[-1].inst[2]))
             fwd.append("}")
+        tile = self.tiled()
+        tiled_src = self._emit_tiles(tile, uses_state) if tile else ""
         desc = {"name": self.name, "real": R, "fields": [n for n, _ in self.fields], "params": self.params,
-                "grid": self.N, "sleeve": s, "block": self.block, "deltas": [list(d) for d in deltas]}
+                "grid": self.N, "sleeve": s, "block": self.block, "deltas": [list(d) for d in deltas],
+                "state_free": not uses_state, "tile": list(tile) if tile else None,
+                "adjoint_tile": list(self.adj_tile) if tile else None}
         flds = "\n".join("#define FLD%d(p) ((%s*)((p) + (size_t)%d * NTOT * sizeof(%s)))" % (i, R, i, R)
                          for i in range(nf))
         lams = "\n".join("#define LAM%d(p) ((%s*)((p) + (size_t)%d * NTOT * sizeof(%s)))" % (i, R, i, R)
                          for i in range(nf))
+        if tile:
+            nt = (self.N[0] // self.adj_tile[0]) * (self.N[1] // self.adj_tile[1])
+            fwd_call = "  fwd_tiles(src, dst_of, nsteps, params_dev, s);"
+            adj_call = "  adj_levels(1, (const char*)state_prev, nullptr, (const char*)lam_next, (char*)lam_prev, params_dev, pgrad, s);"
+            steps_call = ("  if (nsteps > 1) fwd_levels(nsteps - 1, (const char*)state0, (char*)scratch, params_dev, s);\n"
+                          "  adj_levels(nsteps, (const char*)state0, (const char*)scratch, (const char*)lam_next, "
+                          "(char*)lam_prev, params_dev, pgrad, s);")
+            slots = nt
+            levels = self.levels
+        else:
+            fwd_call = ("  for (int t = 0; t < nsteps; ++t) {\n"
+                        "    char* dst = dst_of(nsteps - 1 - t);\n"
+                        "    hipLaunchKernelGGL(k_forward, dim3(nb), dim3(BLOCK), 0, s, src, dst, params_dev);\n"
+                        "    src = dst;\n  }")
+            adj_call = ("  hipLaunchKernelGGL(k_adjoint, dim3(nb), dim3(BLOCK), 0, s, (const char*)state_prev, "
+                        "(const char*)lam_next, (char*)lam_prev, params_dev, pgrad);")
+            steps_call = ("  (void)scratch;\n"
+                          "  hipLaunchKernelGGL(k_adjoint, dim3(nb), dim3(BLOCK), 0, s, (const char*)state0, "
+                          "(const char*)lam_next, (char*)lam_prev, params_dev, pgrad);")
+            slots = None
+            levels = 1
+        desc["levels"] = levels
         return ADJ_TEMPLATE % dict(
-            name=self.name, R=R, nf=nf, np=np_, block=self.block,
+            name=self.name, R=R, nf=nf, np=np_, block=self.block, sleeve=s,
             consts="\n".join("constexpr int N%d = %d;" % (k + 1, n) for k, n in enumerate(self.N)),
             ntot="*".join("(long long)%d" % n for n in self.N), flds=flds, lams=lams,
-            desc=json.dumps(desc), fwd="\n".join("  " + l for l in fwd), adj="\n".join("  " + l for l in body))
+            desc=json.dumps(desc, sort_keys=True), fwd="\n".join("  " + l for l in fwd), adj="\n".join("  " + l for l in body),
+            tiled=tiled_src, fwd_call=fwd_call, adj_call=adj_call, steps_call=steps_call, levels=levels,
+            slots=("%d" % slots) if slots else "(int)((NTOT + BLOCK - 1) / BLOCK)")
+
+    def _emit_tiles(self, tile, uses_state):
+        """LDS-tiled 2-D kernels.
+
+        k_forward_tile<K, ALL>: K steps per launch (tile plus a low-side halo of
+        2*K*s, ping-pong LDS levels), parameters at run time.  With ALL every
+        level's tile is also written out: level j to buffer j-1 (the inner
+        states a fused adjoint needs).
+
+        k_adjoint_levels<K>: K adjoint steps per launch.  Input point q is read
+        by outputs q - delta with delta in [-2s, 0]; they read inputs in
+        [q - 2s, q + 2s].  Reversing K steps for a TX x TY tile therefore needs
+        lam_K on the tile plus a high-side halo of 2*K*s, and the state X_m of
+        level m on [-2s, T + 2(m+1)s) around the tile.  lam lives in two LDS
+        buffers (ping-pong); X_m is loaded into LDS from HBM level by level.
+        The parameter gradients of level m are summed over the tile's own
+        outputs only, so each output counts once."""
+        s = self.plan.sleeve
+        R = self.real
+        nf = len(self.fields)
+        np_ = max(1, len(self.params))
+        tx, ty, kmax = tile
+        ax, ay = self.adj_tile
+        # forward point function on LDS levels
+        pbody = []
+        for si, k in enumerate(self.plan.stores):
+            node = self.plan.mm[k]
+            fb = self._forward_body(k, [0, 0], "s%d_" % si,
+                                    lambda f, a: "lds%d[o + (%d)*RY + (%d)]" % (f, a[0], a[1]))
+            pbody.append("{")
+            pbody += ["  " + l for l in fb.lines]
+            pbody.append("  o%d = s%d_%d;" % (self.fi[node.micro[-1].inst[1]], si, node.micro[-1].inst[2]))
+            pbody.append("}")
+        lparams = ", ".join("const real_t* __restrict__ lds%d" % i for i in range(nf))
+        outs = ", ".join("real_t& o%d" % i for i in range(nf))
+        src = []
+        src.append("template <int RY>\n__device__ __forceinline__ void pt_step(%s, const int o, const double* __restrict__ P, %s) {\n%s\n}\n"
+                   % (lparams, outs, "\n".join("  " + l for l in pbody)))
+        lds = "\n".join("  __shared__ real_t B%d[2][RX * RY];" % i for i in range(nf))
+        load_tile = tile_load("RX * RY", ["const int r = idx / RY, c = idx - r * RY;",
+                                          "const long long g = (long long)wrapn(x0 - H + r, N1) * N2 + wrapn(y0 - H + c, N2);"],
+                              [("B%d[0][idx]" % i, "FLD%d(st_in)[g]" % i) for i in range(nf)], "real_t", "ld")
+        srcs = ", ".join("B%d[sb]" % i for i in range(nf))
+        decl = " ".join("real_t o%d;" % i for i in range(nf))
+        refs = ", ".join("o%d" % i for i in range(nf))
+        st_lds = " ".join("B%d[db][o] = o%d;" % (i, i) for i in range(nf))
+        st_lvl = " ".join("FLD%d(lvl)[g] = o%d;" % (i, i) for i in range(nf))
+        st_glb = " ".join("FLD%d(dst)[g] = o%d;" % (i, i) for i in range(nf))
+        src.append(TILE_FWD % dict(TX=tx, TY=ty, lds=lds, load_tile=load_tile, srcs=srcs, decl=decl, refs=refs,
+                                   st_lds=st_lds, st_lvl=st_lvl, st_glb=st_glb))
+        # adjoint point function on LDS tiles
+        pb, _ = self.point_body(lambda f, a: "X%d[xo + (%d)*XW + (%d)]" % (f, a[0], a[1]),
+                                lambda f, a: "L%d[lo + (%d)*LW + (%d)]" % (f, a[0], a[1]), pguard="own")
+        xs = ", ".join("const real_t* __restrict__ X%d" % i for i in range(nf))
+        ls = ", ".join("const real_t* __restrict__ L%d" % i for i in range(nf))
+        aouts = ", ".join(["real_t& oacc%d" % i for i in range(nf)] + ["double& opg%d" % p for p in range(np_)])
+        fin = ["oacc%d = acc%d;" % (i, i) for i in range(nf)] + ["opg%d = pg%d;" % (p, p) for p in range(np_)]
+        # derivative code only: the compiler may reassociate and contract (hoisting
+        # the products of uniform coefficients); the step's own values stay exact
+        src.append("template <int XW, int LW>\n__device__ __forceinline__ void adj_point(%s, %s, const int xo, const int lo, "
+                   "const bool own, const double* __restrict__ P, %s) {\n"
+                   "#pragma clang fp reassociate(on)\n#pragma clang fp contract(fast)\n%s\n}\n"
+                   % (xs, ls, aouts, "\n".join("  " + l for l in pb + fin)))
+        xl = "\n".join("  __shared__ real_t X%d[XH * XW];" % i for i in range(nf))
+        ll = "\n".join("  __shared__ real_t LA%d[LH * LW];\n  __shared__ real_t LB%d[LH * LW];" % (i, i) for i in range(nf))
+        src.append(TILE_ADJ % dict(
+            TX=ax, TY=ay, xl=xl, ll=ll, uses_state="true" if uses_state else "false",
+            lload=tile_load("LH * LW", ["const int r = idx / LW, c = idx - r * LW;",
+                                        "const long long g = (long long)wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);"],
+                            [("LA%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ll"),
+            xload=tile_load("XHm * XWm", ["const int r = idx / XWm, c = idx - r * XWm;",
+                                          "const long long g = (long long)wrapn(x0 - 2 * SLEEVE + r, N1) * N2 "
+                                          "+ wrapn(y0 - 2 * SLEEVE + c, N2);"],
+                            [("X%d[(idx / XWm) * XW + (idx %% XWm)]" % i, "FLD%d(xs)[g]" % i) for i in range(nf)],
+                            "real_t", "lx", indent="      ", max_count="XH * XW"),
+            sd="\n".join("    const real_t* S%d = (mm & 1) ? LB%d : LA%d; real_t* D%d = (mm & 1) ? LA%d : LB%d;"
+                         % (i, i, i, i, i, i) for i in range(nf)),
+            xargs=", ".join("X%d" % i for i in range(nf)), sargs=", ".join("S%d" % i for i in range(nf)),
+            adecl=" ".join("real_t a%d;" % i for i in range(nf)) + " " + " ".join("double q%d;" % p for p in range(np_)),
+            aargs=", ".join(["a%d" % i for i in range(nf)] + ["q%d" % p for p in range(np_)]),
+            st_lds=" ".join("D%d[lo] = a%d;" % (i, i) for i in range(nf)),
+            st_glb=" ".join("LAM%d(lam_prev)[g] = a%d;" % (i, i) for i in range(nf)),
+            pdecl=" ".join("double pgs%d = 0;" % p for p in range(np_)),
+            pacc=" ".join("pgs%d += q%d;" % (p, p) for p in range(np_)),
+            pgs=", ".join("pgs%d" % p for p in range(np_))))
+        fcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, false>), dim3(NT), dim3(256), 0, s, in, out, P); break;\n"
+                         % (k, k) for k in range(1, kmax + 1))
+        lcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, true>), dim3(NT), dim3(256), 0, s, in, out, P); break;\n"
+                         % (k, k) for k in range(1, self.levels))
+        acases = "".join("    case %d: hipLaunchKernelGGL(k_adjoint_levels<%d>, dim3(NTA), dim3(256), 0, s, st0, lv, lam_next, "
+                         "lam_prev, P, pgrad); break;\n" % (k, k) for k in range(1, self.levels + 1))
+        src.append(TILE_HOST % dict(TX=tx, TY=ty, AX=ax, AY=ay, K=kmax, fcases=fcases, lcases=lcases, acases=acases))
+        return "\n".join(src)
+
+
+TILE_FWD = r"""
+// XCD-aware tile order: workgroup ids go round-robin to the 8 XCDs, so each XCD
+// gets a contiguous band of tiles (shared halos stay in its L2).
+__device__ __forceinline__ int xcd_tile(int b, int nt) { return (nt %% 8 == 0) ? ((b %% 8) * (nt / 8) + b / 8) : b; }
+__device__ __forceinline__ int wrapn(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+template <int K, bool ALL>
+__global__ void __launch_bounds__(256) k_forward_tile(const char* __restrict__ st_in, char* __restrict__ st_out,
+                                                      const double* __restrict__ P) {
+  constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
+  constexpr int NTY = N2 / TY, NT = (N1 / TX) * NTY;
+%(lds)s
+  const int tile = xcd_tile(blockIdx.x, NT);
+  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+%(load_tile)s
+  __syncthreads();
+#pragma unroll
+  for (int j = 1; j <= K; ++j) {
+    const int sb = (j - 1) & 1, db = j & 1;
+    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
+    char* __restrict__ dst = ALL ? st_out + (size_t)(j - 1) * STATE_BYTES : st_out;
+    for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
+      const int rr = idx / W;
+      const int r = lo + rr, c = lo + (idx - rr * W);
+      const int o = r * RY + c;
+      %(decl)s
+      pt_step<RY>(%(srcs)s, o, P, %(refs)s);
+      if (j < K) {
+        %(st_lds)s
+        if (ALL && r >= H && c >= H) {
+          char* __restrict__ lvl = dst;
+          const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+          %(st_lvl)s
+        }
+      } else {
+        const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+        %(st_glb)s
+      }
+    }
+    if (j < K) __syncthreads();
+  }
+}
+"""
+
+TILE_ADJ = r"""
+// K adjoint steps: lam_K (HBM) -> lam_0 (HBM); X_0 = st0, X_m = level buffer m-1 of lv
+template <int K>
+__global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__ st0, const char* __restrict__ lv,
+                                                        const char* __restrict__ lam_next, char* __restrict__ lam_prev,
+                                                        const double* __restrict__ P, double* __restrict__ pgrad) {
+  constexpr int TX = %(TX)d, TY = %(TY)d;
+  constexpr int XH = TX + 2 * (K + 1) * SLEEVE, XW = TY + 2 * (K + 1) * SLEEVE;
+  constexpr int LH = TX + 2 * K * SLEEVE, LW = TY + 2 * K * SLEEVE;
+  constexpr int NTY = N2 / TY, NT = (N1 / TX) * NTY;
+  constexpr bool USES_STATE = %(uses_state)s;
+%(xl)s
+%(ll)s
+  const int tile = xcd_tile(blockIdx.x, NT);
+  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+%(lload)s
+  %(pdecl)s
+#pragma unroll
+  for (int mm = 0; mm < K; ++mm) {
+    const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} -> lam_m at state X_m
+    if (USES_STATE) {
+      const char* __restrict__ xs = m == 0 ? st0 : lv + (size_t)(m - 1) * STATE_BYTES;
+      const int XHm = TX + 2 * (m + 2) * SLEEVE, XWm = TY + 2 * (m + 2) * SLEEVE;
+      __syncthreads();  // every read of the previous level's X is done
+%(xload)s
+    }
+    __syncthreads();
+%(sd)s
+    const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
+    for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
+      const int r = idx / W, c = idx - r * W;
+      const bool own = r < TX && c < TY;
+      %(adecl)s
+      adj_point<XW, LW>(%(xargs)s, %(sargs)s, (r + 2 * SLEEVE) * XW + (c + 2 * SLEEVE), r * LW + c, own, P, %(aargs)s);
+      if (m > 0) {
+        const int lo = r * LW + c;
+        %(st_lds)s
+      } else {
+        const long long g = (long long)(x0 + r) * N2 + (y0 + c);
+        %(st_glb)s
+      }
+      if (own) { %(pacc)s }
+    }
+  }
+  { const double pv[NP] = {%(pgs)s}; block_reduce_params<256>(pgrad + (long long)blockIdx.x * NP, pv); }
+}
+"""
+
+TILE_HOST = r"""
+constexpr int NT = (N1 / %(TX)d) * (N2 / %(TY)d);
+constexpr int NTA = (N1 / %(AX)d) * (N2 / %(AY)d);
+
+// nsteps forward steps, K = %(K)d per launch, ping-pong so that the last lands in dst_of(0)
+template <class DstOf>
+static void fwd_tiles(const char* src, DstOf dst_of, int nsteps, const double* P, hipStream_t s) {
+  int launches = (nsteps + %(K)d - 1) / %(K)d, left = nsteps;
+  for (int l = 0; l < launches; ++l) {
+    const int k = left >= %(K)d ? %(K)d : left;
+    char* out = dst_of(launches - 1 - l);
+    const char* in = src;
+    switch (k) {
+%(fcases)s      default: fmr_rt::fatal("bad step count");
+    }
+    src = out;
+    left -= k;
+  }
+}
+
+// j forward steps from `in`, level i (1..j) written to state buffer i-1 of `out`
+static void fwd_levels(int j, const char* in, char* out, const double* P, hipStream_t s) {
+  switch (j) {
+%(lcases)s    default: fmr_rt::fatal("bad level count");
+  }
+}
+
+static void adj_levels(int k, const char* st0, const char* lv, const char* lam_next, char* lam_prev, const double* P,
+                       double* pgrad, hipStream_t s) {
+  switch (k) {
+%(acases)s    default: fmr_rt::fatal("bad adjoint step count");
+  }
+}
+"""
 
 
 ADJ_TEMPLATE = r'''// Generated by formura_amd: forward and adjoint steps of %(name)s. Do not edit.
@@ -319,9 +615,33 @@ constexpr long long NTOT = %(ntot)s;
 constexpr int NF = %(nf)d;
 constexpr int NP = %(np)d;
 constexpr int BLOCK = %(block)d;
+constexpr int SLEEVE = %(sleeve)d;
+constexpr int KA = %(levels)d;  // most adjoint steps per Formura_AD_Adjoint_Steps call
+constexpr size_t STATE_BYTES = (size_t)NF * NTOT * sizeof(%(R)s);
 %(flds)s
 %(lams)s
 const char* const kDescription = R"FMRJSON(%(desc)s)FMRJSON";
+
+// out[p] += the sum of v[p] over the workgroup: wave sums by shuffles, then the
+// waves' sums in wave order (a fixed order: deterministic, no atomics)
+template <int BS>
+__device__ __forceinline__ void block_reduce_params(double* __restrict__ out, const double* v) {
+  __shared__ double red[NP][BS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    double x = v[p];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[p][w] = x;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < NP) {
+    double t = 0;
+    for (int q = 0; q < BS / 64; ++q) t += red[threadIdx.x][q];
+    out[threadIdx.x] += t;
+  }
+}
 
 __global__ void __launch_bounds__(BLOCK) k_forward(const char* __restrict__ st, char* __restrict__ out,
                                                    const double* __restrict__ P) {
@@ -333,11 +653,13 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const char* __restrict__ st, 
                                                    double* __restrict__ pgrad) {
 %(adj)s
 }
+%(tiled)s
 }  // namespace
 
 extern "C" const char* Formura_AD_Describe(void) { return kDescription; }
 extern "C" long long Formura_AD_State_Bytes(void) { return (long long)NF * NTOT * sizeof(real_t); }
-extern "C" int Formura_AD_Param_Slots(void) { return (int)((NTOT + BLOCK - 1) / BLOCK); }
+extern "C" int Formura_AD_Param_Slots(void) { return %(slots)s; }
+extern "C" int Formura_AD_Levels(void) { return KA; }
 
 // nsteps forward steps from `in`; ping-pong between `out` and `tmp` so that the
 // result lands in `out` (in may alias neither).
@@ -345,25 +667,38 @@ extern "C" void Formura_AD_Forward(const void* in, void* out, void* tmp, int nst
                                    void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const unsigned nb = (unsigned)((NTOT + BLOCK - 1) / BLOCK);
+  (void)nb;
   const char* src = (const char*)in;
-  for (int t = 0; t < nsteps; ++t) {
-    char* dst = ((nsteps - 1 - t) %% 2 == 0) ? (char*)out : (char*)tmp;
-    hipLaunchKernelGGL(k_forward, dim3(nb), dim3(BLOCK), 0, s, src, dst, params_dev);
-    src = dst;
-  }
+  // destination of the launch that has `left` launches after it
+  auto dst_of = [&](int left) { return (left %% 2 == 0) ? (char*)out : (char*)tmp; };
+%(fwd_call)s
   if (nsteps == 0 && in != out)
     FMR_CHECK(hipMemcpyAsync(out, in, (size_t)NF * NTOT * sizeof(real_t), hipMemcpyDeviceToDevice, s));
   FMR_CHECK(hipGetLastError());
 }
 
 // one adjoint step: state_prev is the input of the step, lam_next = dL/d(output);
-// writes lam_prev = dL/d(input) and adds dL/dparams into pgrad[block][param].
+// writes lam_prev = dL/d(input) and adds dL/dparams into pgrad[slot][param].
 extern "C" void Formura_AD_Adjoint(const void* state_prev, const void* lam_next, void* lam_prev,
                                    const double* params_dev, double* pgrad, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const unsigned nb = (unsigned)((NTOT + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_adjoint, dim3(nb), dim3(BLOCK), 0, s, (const char*)state_prev, (const char*)lam_next,
-                     (char*)lam_prev, params_dev, pgrad);
+  (void)nb;
+%(adj_call)s
+  FMR_CHECK(hipGetLastError());
+}
+
+// nsteps (1..Formura_AD_Levels()) adjoint steps in one call: state0 is the input
+// of the first of them, lam_next = dL/d(output of the last); the inner states
+// are recomputed into scratch (nsteps-1 state buffers).  Writes lam_prev =
+// dL/d(state0) and adds dL/dparams of all nsteps steps into pgrad.
+extern "C" void Formura_AD_Adjoint_Steps(const void* state0, void* scratch, const void* lam_next, void* lam_prev,
+                                         int nsteps, const double* params_dev, double* pgrad, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned nb = (unsigned)((NTOT + BLOCK - 1) / BLOCK);
+  (void)nb;
+  if (nsteps < 1 || nsteps > KA) fmr_rt::fatal("Formura_AD_Adjoint_Steps: step count out of range");
+%(steps_call)s
   FMR_CHECK(hipGetLastError());
 }
 '''

@@ -29,7 +29,7 @@ import json
 from typing import Dict, List
 
 from . import syntax as S
-from .emit import Body, EmitError, GraphPlan, state_fields
+from .emit import Body, tile_load, EmitError, GraphPlan, state_fields
 from .manifest import calc_range
 
 PRECISIONS = {"f64": ("double", False), "f32k": ("float", True)}
@@ -97,9 +97,12 @@ class BatchEmitter:
         lds = "\n".join("  __shared__ %s B%d[2][RX * RY];" % (real, i) for i in range(nf))
         if self.kahan:
             lds += "\n" + "\n".join("  __shared__ %s C%d[2][RX * RY];" % (real, i) for i in range(nf))
-        load_tile = "\n".join("      B%d[0][idx] = src[%dLL * plane + g];" % (i, i) for i in range(nf))
+        pairs = [("B%d[0][idx]" % i, "src[%dLL * plane + g]" % i) for i in range(nf)]
         if self.kahan:
-            load_tile += "\n" + "\n".join("      C%d[0][idx] = src[%dLL * plane + g];" % (i, nf + i) for i in range(nf))
+            pairs += [("C%d[0][idx]" % i, "src[%dLL * plane + g]" % (nf + i)) for i in range(nf)]
+        load_tile = tile_load("RX * RY", ["const int r = idx / RY, c = idx - r * RY;",
+                                          "const long long g = (long long)wrapn(tx0 - H + r, I.n1) * I.n2 "
+                                          "+ wrapn(ty0 - H + c, I.n2);"], pairs, real, "ld", indent="    ")
         srcs = ", ".join("B%d[sb]" % i for i in range(nf))
         decl = " ".join("%s old%d, inc%d;" % (real, i, i) for i in range(nf))
         refs = ", ".join("old%d, inc%d" % (i, i) for i in range(nf))
@@ -190,11 +193,7 @@ __global__ void __launch_bounds__(256) k_batch(const Inst* __restrict__ inst, co
     double P[NP];
 #pragma unroll
     for (int q = 0; q < NP; ++q) P[q] = I.p[q];
-    for (int idx = threadIdx.x; idx < RX * RY; idx += 256) {
-      const int r = idx / RY, c = idx - r * RY;
-      const long long g = (long long)wrapn(tx0 - H + r, I.n1) * I.n2 + wrapn(ty0 - H + c, I.n2);
 %(load_tile)s
-    }
     __syncthreads();
 #pragma unroll
     for (int j = 1; j <= K; ++j) {

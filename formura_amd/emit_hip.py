@@ -24,7 +24,7 @@ import json
 from typing import Dict, List
 
 from . import syntax as S
-from .emit import (Body, EmitError, GraphPlan, c_elem_type, emit_header,
+from .emit import (tile_load, Body, EmitError, GraphPlan, c_elem_type, emit_header,
                    lib_description, navi_fields, show_double, state_fields, CTYPE)
 from .manifest import MMNode, calc_range
 
@@ -451,7 +451,11 @@ class Tile2DSchedule:
         wrap1 = em.wrap_fn(0, 2 * self.k * s + 1)
         wrap2 = em.wrap_fn(1, 2 * self.k * s + 1)
         lds = "\n".join("  __shared__ %s B%d[2][RX * RY];" % (ct, i) for i, (n, ct) in enumerate(fields))
-        load_tile = "\n".join("    B%d[0][idx] = FLD%d(st_in)[g];" % (i, i) for i in range(len(fields)))
+        load_tile = tile_load("RX * RY", ["const int r = idx / RY, c = idx - r * RY;",
+                                          "const long long g = (long long)%s(x0 - H + r) * N2 + %s(y0 - H + c);"
+                                          % (wrap1, wrap2)],
+                              [("B%d[0][idx]" % i, "FLD%d(st_in)[g]" % i) for i in range(len(fields))],
+                              fields[0][1], "ld")
         srcs = ", ".join("B%d[src]" % i for i in range(len(fields)))
         decl_o = " ".join("%s o%d;" % (ct, i) for i, (n, ct) in enumerate(fields))
         refs_o = ", ".join("o%d" % i for i in range(len(fields)))
@@ -468,11 +472,7 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
   const int b = blockIdx.x;
   const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
   const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
-  for (int idx = threadIdx.x; idx < RX * RY; idx += 256) {
-    const int r = idx / RY, c = idx - r * RY;
-    const long long g = (long long)%(wrap1)s(x0 - H + r) * N2 + %(wrap2)s(y0 - H + c);
 %(load_tile)s
-  }
   __syncthreads();
 #pragma unroll
   for (int j = 1; j <= K; ++j) {
@@ -560,7 +560,11 @@ class Stream3DSchedule:
         nf = len(fields)
         lds = "\n".join("  __shared__ %s B%d[K][R * PLANE];" % (ct, i) for i, (n, ct) in enumerate(fields))
         wrap = [em.wrap_fn(a, 2 * self.k * s + 1) for a in range(3)]
-        load_plane = "\n".join("      B%d[0][slot * PLANE + idx] = FLD%d(st_in)[g];" % (i, i) for i in range(nf))
+        # software pipeline: plane t+1 is loaded into registers (pf*) while plane t
+        # is computed, and written into its ring slot at the top of the next trip
+        pf_decl = " ".join("%s pf%d[PIT];" % (ct, i) for i, (n, ct) in enumerate(fields))
+        pf_load = "\n".join("          pf%d[it] = FLD%d(st_in)[g];" % (i, i) for i in range(nf))
+        pf_store = "\n".join("        B%d[0][slot * PLANE + idx] = pf%d[it];" % (i, i) for i in range(nf))
         srcs = ", ".join("B%d[j - 1]" % i for i in range(nf))
         decl_o = " ".join("%s o%d;" % (ct, i) for i, (n, ct) in enumerate(fields))
         refs_o = ", ".join("o%d" % i for i in range(nf))
@@ -578,17 +582,33 @@ __global__ void __launch_bounds__(256) k_step_stream(const char* __restrict__ st
   const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
   const int cz = tile %% NTZ, cy = (tile / NTZ) %% NTY, cxi = tile / (NTZ * NTY);
   const int x0 = cxi * CX, y0 = cy * TY, z0 = cz * TZ;
-  for (int t = x0 - H; t < x0 + CX; ++t) {
-    const int slot = ((t %% R) + R) %% R;
-    {
-      const long long gx = (long long)%(w0)s(t) * N2;
-      for (int idx = threadIdx.x; idx < PLANE; idx += 256) {
+  constexpr int PIT = (PLANE + 255) / 256;
+  %(pf_decl)s
+  auto prefetch = [&](int t) {
+    const long long gx = (long long)%(w0)s(t) * N2;
+#pragma unroll
+    for (int it = 0; it < PIT; ++it) {
+      const int idx = threadIdx.x + it * 256;
+      if (idx < PLANE) {
         const int ly = idx / RZ, lz = idx - ly * RZ;
         const long long g = (gx + %(w1)s(y0 - H + ly)) * N3 + %(w2)s(z0 - H + lz);
-%(load_plane)s
+%(pf_load)s
+      }
+    }
+  };
+  prefetch(x0 - H);
+  for (int t = x0 - H; t < x0 + CX; ++t) {
+    const int slot = ((t %% R) + R) %% R;
+    // every read of this ring slot's previous plane ended before the last barrier
+#pragma unroll
+    for (int it = 0; it < PIT; ++it) {
+      const int idx = threadIdx.x + it * 256;
+      if (idx < PLANE) {
+%(pf_store)s
       }
     }
     __syncthreads();
+    if (t + 1 < x0 + CX) prefetch(t + 1);
 #pragma unroll
     for (int j = 1; j <= K; ++j) {
       if (t >= x0 - 2 * (K - j) * SLEEVE) {
@@ -615,7 +635,7 @@ __global__ void __launch_bounds__(256) k_step_stream(const char* __restrict__ st
   }
 }
 """ % dict(TY=self.ty, TZ=self.tz, CX=self.cx, R=R, lds=lds, w0=wrap[0], w1=wrap[1], w2=wrap[2],
-           load_plane=load_plane, srcs=srcs, decl_o=decl_o, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb))
+           pf_decl=pf_decl, pf_load=pf_load, pf_store=pf_store, srcs=srcs, decl_o=decl_o, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb))
         return out
 
     def launch(self, em, plan):

@@ -28,3 +28,15 @@ def test_library_exports_declared_symbols(name):
     syms += re.findall(r"\b(to_pos_\w+)\(", hdr)
     missing = [s for s in syms if not hasattr(lib, s)]
     assert not missing, missing
+
+
+def test_dropin_driver_compiles_against_generated_header():
+    """The example user driver compiles (C11) against the generated header alone."""
+    import subprocess
+    from formura_amd import build
+    d = build.generate_program("heat3d_64", os.path.join(build.PROG_DIR, "heat3d.fmr"),
+                               os.path.join(build.PROG_DIR, "heat3d_64.yaml"), "auto")
+    src = os.path.join(ROOT, "examples", "heat3d_dropin", "main.c")
+    r = subprocess.run(["gcc", "-std=c11", "-Wall", "-fsyntax-only", "-I", d, "-I", os.path.join(ROOT, "include"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

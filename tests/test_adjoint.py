@@ -119,7 +119,8 @@ def test_snapshot_count_does_not_change_gradients():
     for r in res[1:]:
         np.testing.assert_array_equal(r[1], res[0][1])
         np.testing.assert_array_equal(r[2], res[0][2])
-    assert s.last_stats["forward_steps"] == A.forward_steps_needed(steps, 32)
+    assert s.last_stats["forward_steps"] == A.recompute_steps(steps, 32, s.levels)
+    assert s.last_stats["adjoint_steps"] == steps
 
 
 def test_heat_adjoint_matches_torch_cpu_reference():
@@ -146,6 +147,8 @@ def test_heat_adjoint_matches_torch_cpu_reference():
         a = c + (dx * dx / 8) * lap
     ref = 0.5 * ((a - tgt) ** 2).sum()
     ref.backward()
+    # the heat step is linear with no parameters: no checkpoint, no recomputation
+    assert s.state_free and s.last_stats["forward_steps"] == 0
     g = x.grad[0].cpu().numpy()
     assert np.linalg.norm(g - q.grad.numpy()) / np.linalg.norm(q.grad.numpy()) < 1e-12
 
@@ -163,3 +166,34 @@ def test_fp32_adjoint_close_to_fp64():
     g32 = x.grad.double().cpu().numpy()
     err = np.linalg.norm(g32 - gx) / np.linalg.norm(gx)
     assert err < 1e-3, err
+
+
+@pytest.mark.parametrize("name", ["gs_64_adj", "gs_64_adj_f32"])
+def test_fused_adjoint_steps_match_single_steps(name):
+    """One fused call over k steps (inner states recomputed in the call) gives the
+    adjoint of k single steps bit for bit; parameter gradients agree to rounding
+    (their summation order differs)."""
+    s = _stepper(name)
+    st = B.grayscott_init(64, 4)
+    x = torch.tensor(np.stack([st["u"], st["v"]]), device=DEV, dtype=s.dtype)
+    p = torch.tensor(THETA, dtype=torch.float64, device=DEV)
+    g = torch.Generator().manual_seed(2)
+    lam = torch.rand(x.shape, generator=g, dtype=torch.float64).to(DEV, s.dtype)
+    assert s.levels >= 2
+    for k in range(1, s.levels + 1):
+        xs, tmp = [x], torch.empty_like(x)
+        for _ in range(k - 1):
+            y = torch.empty_like(x)
+            s.forward(xs[-1], y, tmp, 1, p)
+            xs.append(y)
+        pg1 = torch.zeros(s.slots, s.nparams, dtype=torch.float64, device=DEV)
+        a, b = lam.clone(), torch.empty_like(lam)
+        for xm in reversed(xs):
+            s.adjoint(xm, a, b, p, pg1)
+            a, b = b, a
+        pg2 = torch.zeros_like(pg1)
+        out = torch.empty_like(lam)
+        s.adjoint_steps(x, s.scratch(DEV), lam, out, k, p, pg2)
+        torch.cuda.synchronize()
+        assert torch.equal(out, a), k
+        torch.testing.assert_close(pg2.sum(0), pg1.sum(0), rtol=1e-12, atol=0)
