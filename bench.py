@@ -124,6 +124,176 @@ def run_ensemble(count: int, steps: int):
     return out
 
 
+def run_per_instance_fp64(sizes, steps: int, probe_steps: int = 600):
+    """The plain per-instance FP64 GPU path for the same ensemble: every instance
+    advanced by its own plain Formura_Forward library (state resident, one
+    library per grid side, ``build.ENSEMBLE_SIDES``).  Each distinct side is
+    timed on one instance over ``probe_steps`` steps; the ensemble time is the
+    sum over instances of (time per step of its side) x ``steps``.  The time per
+    step of a plain Gray-Scott run does not depend on F and k."""
+    from collections import Counter
+    from formura_amd import build as B, inputs
+    from formura_amd.runtime import open_program
+    counts = Counter(int(s[0]) for s in sizes)
+    per, total_ms = {}, 0.0
+    for n, c in sorted(counts.items()):
+        lib = open_program(B.side_name(n))
+        lib.init()
+        lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), 7))
+        lib.set_resident(True)
+        lib.upload()
+        spf = lib.layout["steps_per_forward"]
+        lib.forward_steps(max(1, 30 // spf))
+        k = max(1, probe_steps // spf)
+        ms = lib.forward_steps(k)
+        lib.finalize()
+        per[n] = ms / (k * spf)
+        total_ms += c * per[n] * steps
+    cells = sum(int(a) * int(b) for a, b in sizes)
+    return {"glups": cells * steps / (total_ms * 1e-3) / 1e9, "ms": total_ms, "steps": steps,
+            "sides_timed": len(per), "probe_steps": probe_steps,
+            "us_per_step": {str(n): round(v * 1e3, 3) for n, v in sorted(per.items())}}
+
+
+def run_heat_roofline(name: str = "heat3d_512", steps: int = 200):
+    """Plain FP64 diffusion at full size: GLUPS and the effective HBM fraction
+    (16 B per cell update: read q, write q')."""
+    r = run_gpu(name, steps, 20, 16.0)
+    return {"glups": r["glups"], "schedule": r["schedule"], "bytes_per_lup": 16,
+            "effective_gbs": r["glups"] * 16.0, "effective_frac": r["glups"] * 16.0 / HBM_PEAK_GBS,
+            "mean_launch_us": r["mean_launch_us"], "steps": r["steps"]}
+
+
+def run_euler(steps: int = 200):
+    """3-D Euler HLLC at 256^3: the plain FP64 path and the mixed form (FP32 flux
+    arithmetic; FP64 wave speeds, positivity checks, update and totals) from the
+    same seeded state; error of the mixed run against FP64 per field."""
+    import numpy as np
+    from formura_amd import inputs
+    from formura_amd.runtime import open_program
+    st, out = None, {"steps": steps}
+    res = {}
+    for name in ("euler3d_256", "euler3d_256_mixed"):
+        lib = open_program(name)
+        lib.init()
+        st = st or inputs.euler_state(1, lib.grid)
+        lib.set_state(st)
+        lib.set_resident(True)
+        lib.upload()
+        t0 = lib.totals()
+        ms = lib.forward_steps(steps)
+        t1 = lib.totals()
+        lib.download()
+        res[name] = {f: lib.field(f).copy() for f in lib.fields}
+        cells = int(np.prod(lib.grid))
+        scale = max(abs(v) for v in t0.values())
+        out[name] = {"glups": cells * steps / (ms * 1e-3) / 1e9, "ms": ms, "schedule": lib.layout.get("schedule"),
+                     "max_rel_total_drift": max(abs(t1[f] - t0[f]) for f in t0) / scale,
+                     "positivity_violations": lib.positivity_violations()}
+        lib.set_resident(False)
+        lib.finalize()
+    a, b = res["euler3d_256"], res["euler3d_256_mixed"]
+    out["mixed_rel_maxnorm_err_vs_fp64"] = {f: float(np.max(np.abs(a[f] - b[f])) / np.max(np.abs(a[f]))) for f in a}
+    out["mixed_speedup"] = out["euler3d_256_mixed"]["glups"] / out["euler3d_256"]["glups"]
+    return out
+
+
+def run_adjoint(nsteps: int = 2000, snapshots: int = 32, directions: int = 16, eps: float = 1e-6):
+    """Gray-Scott 1024^2 adjoint: loss 0.5*||v_T - v_target||^2 with v_target from a
+    seeded forward run at (F, k) = (0.035, 0.065); gradients w.r.t. the initial
+    u, v and F, k, Du, Dv by the generated adjoint with binomial checkpointing;
+    checked by FP64 central differences along seeded random directions.  Also
+    the FP32 form (gradient error against FP64) and the 3-D heat adjoint at 256^3."""
+    import numpy as np
+    import torch
+    from formura_amd import adjoint as A, batch as B, inputs
+    dev = "cuda"
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    out = {"config": {"grid": [1024, 1024], "steps": nsteps, "snapshots": snapshots, "directions": directions,
+                      "eps": eps}}
+    st = B.grayscott_init(1024, 11)
+    x0 = np.stack([st["u"], st["v"]])
+    theta = [0.04, 0.06, 2e-5, 1e-5]
+    grads = {}
+    for name, dt in (("gs_1024_adj", torch.float64), ("gs_1024_adj_f32", torch.float32)):
+        s = A.Stepper(name)
+        xd = torch.tensor(x0, device=dev, dtype=dt)
+        with torch.no_grad():
+            vt = A.run(s, xd, torch.tensor([0.035, 0.065, 2e-5, 1e-5], dtype=torch.float64), nsteps, 0)[1].clone()
+        rec = {}
+        for rep in range(2):  # the second repetition is timed warm
+            x = xd.clone().requires_grad_(True)
+            p = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+            torch.cuda.synchronize()
+            e0 = ev()
+            y = A.run(s, x, p, nsteps, snapshots)
+            e1 = ev()
+            loss = 0.5 * ((y[1] - vt) ** 2).sum()
+            e2 = ev()
+            loss.backward()
+            e3 = ev()
+            torch.cuda.synchronize()
+            rec = {"forward_ms": e0.elapsed_time(e1), "backward_ms": e2.elapsed_time(e3),
+                   "ratio": e2.elapsed_time(e3) / e0.elapsed_time(e1), "stats": dict(s.last_stats),
+                   "unit_steps": s.levels}
+        grads[name] = (x.grad.double().cpu().numpy(), p.grad.cpu().numpy(), s, vt)
+        out[name] = rec
+    gx, gp, s, vt = grads["gs_1024_adj"]
+    rng = np.random.default_rng(2024)
+
+    def L(x, th):
+        with torch.no_grad():
+            y = A.run(s, torch.tensor(x, device=dev), torch.tensor(th, dtype=torch.float64), nsteps, 0)
+            return 0.5 * float(((y[1] - vt) ** 2).sum())
+
+    errs = []
+    for _ in range(directions):
+        dx = rng.standard_normal(x0.shape)
+        dp = rng.standard_normal(4) * np.array(theta)
+        fd = (L(x0 + eps * dx, np.array(theta) + eps * dp) - L(x0 - eps * dx, np.array(theta) - eps * dp)) / (2 * eps)
+        an = float((gx * dx).sum() + (gp * dp).sum())
+        errs.append(abs(fd - an) / max(abs(an), 1e-300))
+    out["gs_1024_adj"]["fd_rel_err"] = {"max": max(errs), "median": float(np.median(errs))}
+    g32x, g32p = grads["gs_1024_adj_f32"][:2]
+    out["gs_1024_adj_f32"]["grad_rel_l2_vs_fp64"] = {
+        "state": float(np.linalg.norm(g32x - gx) / np.linalg.norm(gx)),
+        "params": [float(abs(a - b) / abs(b)) for a, b in zip(g32p, gp)]}
+    # 3-D heat: linear step without parameters, no checkpoint needed
+    s = A.Stepper("heat3d_256_adj")
+    q0 = inputs.uniform(5, (256, 256, 256))[None]
+    tgt = torch.tensor(inputs.uniform(6, (256, 256, 256)), device=dev)
+    hsteps = 100
+    x = torch.tensor(q0, device=dev, requires_grad=True)
+    torch.cuda.synchronize()
+    e0 = ev()
+    y = A.run(s, x, torch.zeros(0, dtype=torch.float64), hsteps, snapshots)
+    e1 = ev()
+    loss = 0.5 * ((y[0] - tgt) ** 2).sum()
+    e2 = ev()
+    loss.backward()
+    e3 = ev()
+    torch.cuda.synchronize()
+    g = x.grad.cpu().numpy()
+    herr = []
+    for _ in range(4):
+        d = rng.standard_normal(q0.shape)
+        with torch.no_grad():
+            lp = 0.5 * float(((A.run(s, torch.tensor(q0 + eps * d, device=dev), torch.zeros(0, dtype=torch.float64), hsteps, 0)[0] - tgt) ** 2).sum())
+            lm = 0.5 * float(((A.run(s, torch.tensor(q0 - eps * d, device=dev), torch.zeros(0, dtype=torch.float64), hsteps, 0)[0] - tgt) ** 2).sum())
+        fd = (lp - lm) / (2 * eps)
+        an = float((g * d).sum())
+        herr.append(abs(fd - an) / abs(an))
+    out["heat3d_256_adj"] = {"steps": hsteps, "forward_ms": e0.elapsed_time(e1), "backward_ms": e2.elapsed_time(e3),
+                             "ratio": e2.elapsed_time(e3) / e0.elapsed_time(e1), "stats": dict(s.last_stats),
+                             "fd_rel_err_max": max(herr)}
+    return out
+
+
 def cpu_baseline(name: str, target_s: float = 10.0):
     """Oracle (fast build) on the host cores, timed on a bounded number of steps."""
     import numpy as np
@@ -168,6 +338,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=1024,
                     help="instances of the batched FP32 Gray-Scott ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the heat 512^3, Euler 256^3 and adjoint legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB, 20 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -182,6 +354,13 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": a.numel(), "launches": 20}))
         return
+    # torch first: its HIP runtime is then the one every library of this process
+    # binds to (the adjoint leg runs through torch)
+    try:
+        import torch
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the periodic halo is wrapped in HBM; "
               "no inter-GPU exchange is built). Use --gpus 1.", file=sys.stderr)
@@ -215,10 +394,21 @@ def main():
     if args.ensemble > 0:
         try:
             ens = run_ensemble(args.ensemble, args.ensemble_steps)
-            ens["fp32_batched_vs_plain_fp64"] = ens["gs_batch_f32k"]["glups"] / r["glups"]
+            from formura_amd import batch as B
+            sizes, _, _ = B.grayscott_ensemble(args.ensemble)
+            pi = run_per_instance_fp64(sizes, args.ensemble_steps)
+            ens["per_instance_fp64"] = pi
+            ens["fp32_batched_vs_per_instance_fp64"] = ens["gs_batch_f32k"]["glups"] / pi["glups"]
+            ens["fp32_batched_vs_plain_fp64_2048"] = ens["gs_batch_f32k"]["glups"] / r["glups"]
             line["ensemble"] = ens
         except Exception as e:
             line["ensemble"] = {"error": str(e)[:300]}
+    if not args.no_extras:
+        for key, fn in (("heat3d_512", run_heat_roofline), ("euler3d_256", run_euler), ("adjoint", run_adjoint)):
+            try:
+                line[key] = fn()
+            except Exception as e:
+                line[key] = {"error": str(e)[:300]}
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         try:
             line["cpu_baseline"] = cpu_baseline(name, args.cpu_seconds)

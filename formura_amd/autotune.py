@@ -13,6 +13,11 @@ Two phases, so that compilation stays off the GPU box:
 
 Copying that JSON to ``formura_amd/tuning/NAME.json`` makes the build use the
 winner (``emit_hip.choose_schedule``).
+
+Batched ensembles (``build.BATCHES``) are tuned the same way over tile shape,
+steps per launch and persistent grid size (``build_batch_variants`` /
+``run_batch_variants``), on a seeded ensemble of the same size distribution;
+every variant's final states must be bit-identical to the first variant's.
 """
 from __future__ import annotations
 
@@ -125,6 +130,73 @@ def run_variants(name: str, out_dir: str, seconds: float = 0.3, seed: int = 5):
     return out
 
 
+def batch_candidates(precision: str):
+    """Tile (tx, ty), steps per launch k and persistent grid size for a batch
+    library; ty and tx divide every side of the ensemble (multiples of 32) and
+    the LDS of a workgroup stays within 64 KiB."""
+    arrays, size = (4, 4) if precision == "f32k" else (2, 8)
+    out = []
+    for tx, ty in ((16, 32), (32, 32), (8, 32), (16, 16), (32, 16)):
+        for k in (2, 3, 4, 5, 6, 8):
+            lds = (tx + 2 * k) * (ty + 2 * k) * arrays * 2 * size
+            if lds > 64 * 1024:
+                continue
+            for grid in (1024, 2048, 4096):
+                out.append({"tx": tx, "ty": ty, "k": k, "grid": grid})
+    return out
+
+
+def build_batch_variants(name: str, jobs: int = 8):
+    prec = B.BATCHES[name][2]
+    todo = []
+    for spec in batch_candidates(prec):
+        d = os.path.join(variant_dir(name), _tag(spec))
+        os.makedirs(d, exist_ok=True)
+        with open(os.path.join(d, "spec.json"), "w") as f:
+            json.dump(spec, f)
+        todo.append((spec, d))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda a: B.build_batch(name, spec=a[0], out_dir=a[1]), todo))
+    return [d for _, d in todo]
+
+
+def run_batch_variants(name: str, out_dir: str, count: int = 256, steps: int = 480, seed: int = 77):
+    import numpy as np
+    from . import batch as BT
+    sizes, params, seeds = BT.grayscott_ensemble(count, seed=seed)
+    init = [BT.grayscott_init(sz[0], sd) for sz, sd in zip(sizes, seeds)]
+    root = variant_dir(name)
+    dirs = sorted(d for d in os.listdir(root) if os.path.exists(os.path.join(root, d, "lib%s.so" % name)))
+    probe = [0, count // 3, count - 1]
+    results, ref, ref_tag = [], None, None
+    for d in dirs:
+        spec = json.load(open(os.path.join(root, d, "spec.json")))
+        e = BT.Ensemble(name, sizes, params, path=os.path.join(root, d, "lib%s.so" % name))
+        for i, st in enumerate(init):
+            e.set_state(i, st)
+        e.step(48)  # warm-up: code objects, graph capture
+        for i, st in enumerate(init):  # every variant then runs the same steps from the same state
+            e.set_state(i, st)
+        ms = e.step(steps)
+        got = [e.get_state(i) for i in probe]
+        if ref is None:
+            ref, ref_tag, same = got, d, True
+        else:
+            same = all(np.array_equal(a[f], b[f]) for a, b in zip(got, ref) for f in a)
+        glups = e.cells() * steps / (ms * 1e-3) / 1e9
+        e.close()
+        results.append({"spec": spec, "tag": d, "glups": glups, "ms": ms, "steps": steps, "identical_to_first": bool(same)})
+        print("%-28s %8.2f GLUPS  %s" % (d, glups, "ok" if same else "MISMATCH"), flush=True)
+    good = [r for r in results if r["identical_to_first"]]
+    best = max(good, key=lambda r: r["glups"]) if good else None
+    out = {"name": name, "best": best["spec"] if best else None, "results": results,
+           "reference_variant": ref_tag, "ensemble": {"count": count, "seed": seed, "steps": steps}}
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, "%s.json" % name), "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("phase", choices=["build", "run"])
@@ -133,8 +205,11 @@ def main():
     ap.add_argument("--seconds", type=float, default=0.3)
     a = ap.parse_args()
     for n in a.names:
+        batch = n in B.BATCHES
         if a.phase == "build":
-            print(n, len(build_variants(n)), "variants", flush=True)
+            print(n, len((build_batch_variants if batch else build_variants)(n)), "variants", flush=True)
+        elif batch:
+            run_batch_variants(n, a.out)
         else:
             run_variants(n, a.out, a.seconds)
 

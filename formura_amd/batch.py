@@ -23,8 +23,8 @@ from . import inputs
 
 
 class Ensemble:
-    def __init__(self, name: str, sizes: Sequence[tuple], params: np.ndarray):
-        path = _build.lib_path(name)
+    def __init__(self, name: str, sizes: Sequence[tuple], params: np.ndarray, path: str = None):
+        path = path or _build.lib_path(name)
         if not os.path.exists(path):
             raise RuntimeError("batch library %s is missing at %s; build it with "
                                "`python -m formura_amd.build %s`" % (name, path, name))

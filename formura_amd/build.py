@@ -54,6 +54,13 @@ PROGRAMS: Dict[str, tuple] = {
     "euler3d_32_inline_stream": ("euler3d.fmr", "euler3d_32.yaml",
                                  {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 1, "inline": 1}),
     "gs_64_tile_k8": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 8}),
+    # mixed precision: FP32 flux arithmetic, FP64 wave speeds, positivity checks and update
+    "euler3d_16_mixed": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),
+    "euler3d_32_mixed": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),
+    "euler3d_32_mixed_stream": ("euler3d.fmr", "euler3d_32.yaml",
+                                {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 1, "inline": 1, "mixed": 1}),
+    "euler3d_256_mixed": ("euler3d.fmr", "euler3d_256.yaml",
+                          {"kind": "stream3d", "ty": 8, "tz": 32, "cx": 32, "k": 1, "inline": 1, "mixed": 1}),
 }
 
 
@@ -70,10 +77,12 @@ def build_batch(name: str, force: bool = False, spec: Optional[dict] = None, out
     fmr, yml, prec, params, tx, ty, k = BATCHES[name]
     if spec is None:
         spec = _tuned_batch(name)
+    grid = 2048
     if spec:
         tx, ty, k = int(spec["tx"]), int(spec["ty"]), int(spec["k"])
+        grid = int(spec.get("grid", grid))
     _, _, mm = generate(os.path.join(PROG_DIR, fmr), os.path.join(PROG_DIR, yml), name, "flat")
-    src = BatchEmitter(mm, name, prec, params, tx, ty, k).emit()
+    src = BatchEmitter(mm, name, prec, params, tx, ty, k, grid_wgs=grid).emit()
     d = out_dir or os.path.join(BUILD_DIR, name)
     os.makedirs(d, exist_ok=True)
     p = os.path.join(d, "%s.hip" % name)
@@ -124,6 +133,30 @@ def _tuned_batch(name: str):
     return None
 
 
+# Plain single-instance Gray-Scott programs for every side of the batched
+# ensemble (64..1024 in steps of 32): the per-instance FP64 GPU path that the
+# batched ensemble is compared with.  Schedule: the one tuned for 2048^2.
+ENSEMBLE_SIDES = tuple(range(64, 1025, 32))
+SIDE_SPEC = {"kind": "tile2d", "tx": 16, "ty": 32, "k": 3}
+
+
+def side_name(n: int) -> str:
+    return "gs_side_%d" % n
+
+
+def build_side(name: str, force: bool = False) -> str:
+    n = int(name.rsplit("_", 1)[1])
+    d = os.path.join(BUILD_DIR, name)
+    os.makedirs(d, exist_ok=True)
+    y = os.path.join(d, "%s.yaml" % name)
+    text = "length_per_node: [%d.%02d,%d.%02d]\ngrid_per_node: [%d,%d]\n" % (n // 100, n % 100, n // 100, n % 100, n, n)
+    if not os.path.exists(y) or open(y).read() != text:
+        with open(y, "w") as f:
+            f.write(text)
+    generate_program(name, os.path.join(PROG_DIR, "grayscott2d.fmr"), y, dict(SIDE_SPEC), out_dir=d)
+    return compile_library(name, d, force=force)
+
+
 def lib_path(name: str) -> str:
     return os.path.join(BUILD_DIR, name, "lib%s.so" % name)
 
@@ -172,12 +205,21 @@ def build_one(name: str, force: bool = False) -> str:
 
 
 def build_all(names: Optional[List[str]] = None, force: bool = False, jobs: int = 0) -> List[str]:
-    names = names or (list(PROGRAMS) + list(BATCHES) + list(ADJOINTS))
+    names = names or (list(PROGRAMS) + list(BATCHES) + list(ADJOINTS) + [side_name(n) for n in ENSEMBLE_SIDES])
     jobs = jobs or min(8, os.cpu_count() or 4)
     outs = []
+
+    def builder(n):
+        if n in BATCHES:
+            return build_batch
+        if n in ADJOINTS:
+            return build_adjoint
+        if n.startswith("gs_side_"):
+            return build_side
+        return build_one
+
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = {ex.submit(build_batch if n in BATCHES else (build_adjoint if n in ADJOINTS else build_one), n, force): n
-                for n in names}
+        futs = {ex.submit(builder(n), n, force): n for n in names}
         for f in cf.as_completed(futs):
             outs.append(f.result())
     return outs

@@ -40,7 +40,39 @@ __device__ __forceinline__ int fmr_mod(int i) {
   return r < 0 ? r + N : r;
 }
 
+// Positivity checks of the mixed-precision form: every FP64 square-root
+// argument (e.g. gamma*p/rho of a wave speed) must be > 0; a non-positive or
+// NaN argument is counted (vector atomic, taken only on a violation).
+__device__ unsigned long long fmr_violations = 0;
+#define FMR_POSITIVE(x) \
+  do { \
+    if (!((x) > 0)) atomicAdd(&fmr_violations, 1ull); \
+  } while (0)
+
 namespace fmr_rt {
+
+// FP64 totals of a field, in a fixed order (deterministic): TOTALS_BLOCKS
+// workgroups sum fixed grid-stride subsets (shuffles within a wave, then the
+// waves in order); one wave adds the partial sums in order.
+constexpr int TOTALS_BLOCKS = 1024;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_totals_partial(const T *__restrict__ f, long long n, double *__restrict__ part) {
+  double s = 0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) s += (double)f[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double w[4];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (w[0] + w[1]) + (w[2] + w[3]);
+}
+
+__global__ void __launch_bounds__(64) k_totals_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += 64) s += part[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (threadIdx.x == 0) *out = s;
+}
 
 struct Runtime {
   int ready = 0;
@@ -52,6 +84,8 @@ struct Runtime {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   long long launches = 0;  // kernel launches in the last Formura_Forward_Steps
+  double *scratch = nullptr;  // totals: partial sums and results
+  size_t scratch_n = 0;
   void swap() {
     char *t = cur;
     cur = nxt;
@@ -103,11 +137,37 @@ inline void timer_stop(Runtime &rt) {
   rt.last_ms = ms;
 }
 
+template <typename T>
+inline void field_total(Runtime &rt, const T *f, long long n, double *dev_out) {
+  if (rt.scratch_n < (size_t)TOTALS_BLOCKS) fatal("field_total: scratch not allocated");
+  hipLaunchKernelGGL(k_totals_partial<T>, dim3(TOTALS_BLOCKS), dim3(256), 0, rt.stream, f, n, rt.scratch);
+  hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(64), 0, rt.stream, rt.scratch, TOTALS_BLOCKS, dev_out);
+  FMR_CHECK(hipGetLastError());
+}
+
+inline double *scratch(Runtime &rt, size_t n) {
+  if (rt.scratch_n < n) {
+    FMR_CHECK(hipStreamSynchronize(rt.stream));
+    device_free(rt.scratch);
+    rt.scratch = (double *)device_alloc(n * sizeof(double));
+    rt.scratch_n = n;
+  }
+  return rt.scratch;
+}
+
+inline long long violations(Runtime &rt) {
+  if (rt.ready) FMR_CHECK(hipStreamSynchronize(rt.stream));
+  unsigned long long v = 0;
+  FMR_CHECK(hipMemcpyFromSymbol(&v, HIP_SYMBOL(fmr_violations), sizeof(v), 0, hipMemcpyDeviceToHost));
+  return (long long)v;
+}
+
 inline void shutdown(Runtime &rt) {
   if (!rt.ready) return;
   FMR_CHECK(hipStreamSynchronize(rt.stream));
   device_free(rt.cur);
   device_free(rt.nxt);
+  device_free(rt.scratch);
   FMR_CHECK(hipEventDestroy(rt.ev0));
   FMR_CHECK(hipEventDestroy(rt.ev1));
   FMR_CHECK(hipStreamDestroy(rt.stream));

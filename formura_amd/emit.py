@@ -184,21 +184,30 @@ class Body:
 
     def __init__(self, node: MMNode, load: Callable, tmp_load: Callable, index: Callable,
                  store: Callable, prefix: str, fold: bool = True, const_kw="const ",
-                 params: Dict[str, str] = None, real: str = None, recip: bool = False):
+                 params: Dict[str, str] = None, real: str = None, recip: bool = False,
+                 prec: Dict[int, str] = None):
         """``params``: literal names bound to run-time expressions (not folded);
         ``real``: if set ("float"), every floating micro value is computed in
         that type (the FP32 form; constants are folded in double and rounded,
         and a division by a constant becomes a multiplication by its reciprocal);
         ``recip``: divide by constants through their reciprocal in double too
-        (not bit-exact: for derivative code, never for a step's own values)."""
+        (not bit-exact: for derivative code, never for a step's own values);
+        ``prec``: per-micro C type of the floating values ("double"/"float",
+        from ``mixed_precision``); operands of another precision are converted,
+        FP32 values divide by constants through the reciprocal, and every
+        FP64 square-root argument is checked to be positive (FMR_POSITIVE)."""
         self.lines: List[str] = []
         vals: Dict[int, object] = {}
         params = params or {}
+        ctypes: Dict[int, str] = {}
         for j, m in enumerate(node.micro):
             typ = m.typ
             ct = CTYPE.get(typ, typ)
             if real and ct == "double":
                 ct = real
+            if prec and ct == "double" and j in prec:
+                ct = prec[j]
+            ctypes[j] = ct
             name = "%s%d" % (prefix, j)
             k = m.inst[0]
             if k == "Store":
@@ -219,10 +228,14 @@ class Body:
                 self.lines.append("%s%s %s = (%s%d - %s%d) + (%s%d - %s%d);"
                                   % (const_kw, ct, name, prefix, pa, prefix, pc, prefix, pb, prefix, pc))
                 continue
-            if (real or recip) and k == "Binop" and m.inst[1] == "/" and m.inst[3] in vals and m.inst[2] not in vals:
+            if ((real or recip or ct == "float") and k == "Binop" and m.inst[1] == "/" and m.inst[3] in vals
+                    and m.inst[2] not in vals):
                 rc = 1.0 / vals[m.inst[3]] if vals[m.inst[3]] else float("inf")
-                lit = _const_literal(_f32(rc), "float") if real == "float" else show_double(rc)
-                self.lines.append("%s%s %s = %s%d*%s;" % (const_kw, ct, name, prefix, m.inst[2], lit))
+                lit = _const_literal(_f32(rc), "float") if ct == "float" else show_double(rc)
+                src = "%s%d" % (prefix, m.inst[2])
+                if prec and ctypes.get(m.inst[2]) in ("float", "double") and ctypes[m.inst[2]] != ct:
+                    src = "(%s)%s" % (ct, src)
+                self.lines.append("%s%s %s = %s*%s;" % (const_kw, ct, name, src, lit))
                 continue
             if fold:
                 ops = _operands(m)
@@ -230,11 +243,21 @@ class Body:
                     v = _fold(m, vals, typ)
                     if v is not None:
                         vals[j] = v
-                        lit = _const_literal(_f32(v) if (real == "float" and isinstance(v, float)) else v,
-                                             "float" if (real == "float" and isinstance(v, float)) else typ)
+                        f32 = (real == "float" or ct == "float") and isinstance(v, float)
+                        lit = _const_literal(_f32(v) if f32 else v, "float" if f32 else typ)
                         self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, lit))
                         continue
-            a = lambda i: "%s%d" % (prefix, i)
+            def a(i, ct=ct, k=k):
+                v = "%s%d" % (prefix, i)
+                if prec and ct in ("float", "double") and ctypes.get(i) in ("float", "double") and ctypes[i] != ct:
+                    return "(%s)%s" % (ct, v)
+                return v
+            if prec and k == "Binop" and m.inst[1] in ("<", "<=", ">", ">=", "==", "!="):
+                # a comparison is evaluated in the wider type of its operands
+                wide = "double" if "double" in (ctypes.get(m.inst[2]), ctypes.get(m.inst[3])) else ct
+                a = lambda i, wide=wide: ("(%s)%s%d" % (wide, prefix, i)
+                                          if ctypes.get(i) in ("float", "double") and ctypes[i] != wide
+                                          else "%s%d" % (prefix, i))
             if k == "Imm":
                 rhs = show_double(float(Fraction(m.inst[1])))
                 if real == "float":
@@ -248,22 +271,81 @@ class Body:
             elif k == "Uniop":
                 op = m.inst[1]
                 if op.startswith("external-call/"):
-                    rhs = "%s((%s)%s)" % (op.split("/", 1)[1], real or "double", a(m.inst[2]))
+                    rhs = "%s((%s)%s)" % (op.split("/", 1)[1], ct if prec else (real or "double"), a(m.inst[2]))
+                    if prec and op == "external-call/sqrt" and ct == "double":
+                        self.lines.append("FMR_POSITIVE(%s);" % a(m.inst[2]))
                 else:
                     rhs = "%s%s" % (op, a(m.inst[2]))
             elif k == "Binop":
                 op = m.inst[1]
                 if op == "**":
-                    rhs = "pow((%s)%s,(%s)%s)" % (real or "double", a(m.inst[2]), real or "double", a(m.inst[3]))
+                    pt = ct if prec else (real or "double")
+                    rhs = "pow((%s)%s,(%s)%s)" % (pt, a(m.inst[2]), pt, a(m.inst[3]))
                 else:
                     rhs = "%s%s%s" % (a(m.inst[2]), op, a(m.inst[3]))
             elif k == "Triop":
-                rhs = "%s?%s:%s" % (a(m.inst[2]), a(m.inst[3]), a(m.inst[4]))
+                rhs = "%s%d?%s:%s" % (prefix, m.inst[2], a(m.inst[3]), a(m.inst[4]))
             else:
                 raise EmitError("Unimplemented for keyword: %s" % k)
             self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, rhs))
         self.result = "%s%d" % (prefix, len(node.micro) - 1)
         self.vals = vals
+
+
+COMPARISONS = ("<", "<=", ">", ">=", "==", "!=")
+
+
+def mixed_precision(node: MMNode) -> Dict[int, str]:
+    """Precision of each floating micro value of a node in the mixed form.
+
+    FP64: (1) every value that decides a branch or feeds a square root, with
+    all its operands -- for an approximate Riemann solver the wave-speed
+    estimates and the pressures and densities they come from; (2) the loads of
+    the (FP64) state; (3) the update chain: from the stored value down through
+    + - * / and unary minus, so that q' = q - dt*(F_i - F_{i-1})/dx is summed in
+    FP64 (flux differences telescope, totals are conserved to FP64 rounding).
+    FP32: everything else (the flux arithmetic)."""
+    micro = node.micro
+    fp64 = [False] * len(micro)
+
+    def mark(i):
+        stack = [i]
+        while stack:
+            j = stack.pop()
+            if j is None or fp64[j]:
+                continue
+            fp64[j] = True
+            stack += [o for o in _operands(micro[j]) if o is not None]
+
+    for j, m in enumerate(micro):
+        k = m.inst[0]
+        if k == "Binop" and m.inst[1] in COMPARISONS:
+            mark(j)
+        elif k == "Triop":
+            mark(m.inst[2])
+        elif k == "Uniop" and m.inst[1] == "external-call/sqrt":
+            mark(j)
+        elif k in ("LoadCursorStatic", "LoadCursor"):
+            fp64[j] = True
+    # constants (and values computed from constants only) keep their exact FP64
+    # value; FP32 consumers convert them (at compile time)
+    const = [False] * len(micro)
+    for j, m in enumerate(micro):
+        ops = _operands(m)
+        const[j] = m.inst[0] == "Imm" or (m.inst[0] in ("Uniop", "Binop", "Triop") and all(
+            o is not None and const[o] for o in ops))
+        if const[j]:
+            fp64[j] = True
+    st = micro[-1]
+    if st.inst[0] == "Store":
+        stack = [st.inst[2]]
+        while stack:
+            j = stack.pop()
+            m = micro[j]
+            if (m.inst[0] == "Binop" and m.inst[1] in ("+", "-", "*", "/")) or (m.inst[0] == "Uniop" and m.inst[1] == "-"):
+                fp64[j] = True
+                stack += _operands(m)
+    return {j: ("double" if fp64[j] else "float") for j in range(len(micro))}
 
 
 def _operands(m):

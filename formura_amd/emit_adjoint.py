@@ -451,11 +451,10 @@ class AdjointEmitter:
             lload=tile_load("LH * LW", ["const int r = idx / LW, c = idx - r * LW;",
                                         "const long long g = (long long)wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);"],
                             [("LA%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ll"),
-            xload=tile_load("XHm * XWm", ["const int r = idx / XWm, c = idx - r * XWm;",
-                                          "const long long g = (long long)wrapn(x0 - 2 * SLEEVE + r, N1) * N2 "
-                                          "+ wrapn(y0 - 2 * SLEEVE + c, N2);"],
-                            [("X%d[(idx / XWm) * XW + (idx %% XWm)]" % i, "FLD%d(xs)[g]" % i) for i in range(nf)],
-                            "real_t", "lx", indent="      ", max_count="XH * XW"),
+            xdecl=" ".join("real_t nx%d[NX_IT];" % i for i in range(nf)),
+            xfetch="\n".join("          nx%d[it] = FLD%d(xs)[g];" % (i, i) for i in range(nf)),
+            xput="\n".join("        X%d[(idx / XWm) * XW + (idx - (idx / XWm) * XWm)] = nx%d[it];" % (i, i)
+                           for i in range(nf)),
             sd="\n".join("    const real_t* S%d = (mm & 1) ? LB%d : LA%d; real_t* D%d = (mm & 1) ? LA%d : LB%d;"
                          % (i, i, i, i, i, i) for i in range(nf)),
             xargs=", ".join("X%d" % i for i in range(nf)), sargs=", ".join("S%d" % i for i in range(nf)),
@@ -535,18 +534,42 @@ __global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__
 %(ll)s
   const int tile = xcd_tile(blockIdx.x, NT);
   const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+  // the state tile of the next level is fetched into registers (nx*) while the
+  // current level computes; it enters LDS once every read of the current one ended
+  constexpr int NX_IT = (XH * XW + 255) / 256;
+  %(xdecl)s
+  auto xfetch = [&](int m) {
+    const char* __restrict__ xs = m == 0 ? st0 : lv + (size_t)(m - 1) * STATE_BYTES;
+    const int XHm = TX + 2 * (m + 2) * SLEEVE, XWm = TY + 2 * (m + 2) * SLEEVE;
+#pragma unroll
+    for (int it = 0; it < NX_IT; ++it) {
+      const int idx = threadIdx.x + it * 256;
+      if (idx < XHm * XWm) {
+        const int r = idx / XWm, c = idx - r * XWm;
+        const long long g = (long long)wrapn(x0 - 2 * SLEEVE + r, N1) * N2 + wrapn(y0 - 2 * SLEEVE + c, N2);
+%(xfetch)s
+      }
+    }
+  };
+  if (USES_STATE) xfetch(K - 1);
 %(lload)s
   %(pdecl)s
 #pragma unroll
   for (int mm = 0; mm < K; ++mm) {
     const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} -> lam_m at state X_m
     if (USES_STATE) {
-      const char* __restrict__ xs = m == 0 ? st0 : lv + (size_t)(m - 1) * STATE_BYTES;
       const int XHm = TX + 2 * (m + 2) * SLEEVE, XWm = TY + 2 * (m + 2) * SLEEVE;
       __syncthreads();  // every read of the previous level's X is done
-%(xload)s
+#pragma unroll
+      for (int it = 0; it < NX_IT; ++it) {
+        const int idx = threadIdx.x + it * 256;
+        if (idx < XHm * XWm) {
+%(xput)s
+        }
+      }
     }
     __syncthreads();
+    if (USES_STATE && m > 0) xfetch(m - 1);
 %(sd)s
     const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
     for (int idx = threadIdx.x; idx < HH * W; idx += 256) {

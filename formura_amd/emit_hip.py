@@ -24,7 +24,7 @@ import json
 from typing import Dict, List
 
 from . import syntax as S
-from .emit import (tile_load, Body, EmitError, GraphPlan, c_elem_type, emit_header,
+from .emit import (tile_load, mixed_precision, Body, EmitError, GraphPlan, c_elem_type, emit_header,
                    lib_description, navi_fields, show_double, state_fields, CTYPE)
 from .manifest import MMNode, calc_range
 
@@ -56,10 +56,15 @@ class LibEmitter:
             from .emit_hip import _tuned_spec
             spec = _tuned_spec(name)
         self.inlined = bool(isinstance(spec, dict) and spec.get("inline"))
+        # mixed precision: FP32 arithmetic except branch decisions, square-root
+        # arguments and the update chain (emit.mixed_precision), decided per
+        # node before any inlining
+        self.mixed = bool(isinstance(spec, dict) and spec.get("mixed"))
+        self._prec = {k: mixed_precision(n) for k, n in mm.step.items()} if self.mixed else None
         if self.inlined:
             from .manifest import inline_temporaries
             orig_rng = calc_range(mm.step)
-            step_mm = inline_temporaries(mm.step)
+            step_mm = inline_temporaries(mm.step, prec=self._prec)
             new_rng = calc_range(step_mm)
             for k in step_mm:
                 if (orig_rng[k].lower, orig_rng[k].upper) != (new_rng[k].lower, new_rng[k].upper):
@@ -75,6 +80,13 @@ class LibEmitter:
         self.schedule = choose_schedule(self, schedule)
 
     # -- helpers -----------------------------------------------------------------
+    def prec_of(self, node):
+        if not self.mixed:
+            return None
+        if any(m.prec for m in node.micro):  # inlined: precision carried from the source nodes
+            return {j: m.prec for j, m in enumerate(node.micro) if m.prec}
+        return self._prec.get(node.omid) or mixed_precision(node)
+
     def wrap_fn(self, k, reach):
         n = self.N[k]
         if reach < n:
@@ -146,7 +158,7 @@ class LibEmitter:
             def index(ax):
                 return "(%s+off_%d)" % (ivs[ax], ax + 1)
 
-            body = Body(node, load, tmp_load, index, store, "s%d_" % k)
+            body = Body(node, load, tmp_load, index, store, "s%d_" % k, prec=self.prec_of(node))
             lines.append("{")
             lines += ["  " + l for l in body.lines]
             lines.append("}")
@@ -365,7 +377,19 @@ extern "C" void *Formura_HIP_State_Device(void) { return g_rt.cur; }
 extern "C" void *Formura_HIP_Stream(void) { return (void*)g_rt.stream; }
 extern "C" double Formura_HIP_Last_Elapsed_Ms(void) { return g_rt.last_ms; }
 extern "C" long long Formura_HIP_Last_Launches(void) { return g_rt.launches; }
-""" % dict(advance=self.schedule.advance(self), forward_body=self.forward_body(shift, adv_filter),
+
+extern "C" void Formura_HIP_Totals(double *out) {
+  if (!g_rt.ready) fmr_rt::fatal("Formura_HIP_Totals called before Formura_Init");
+  double *res = fmr_rt::scratch(g_rt, fmr_rt::TOTALS_BLOCKS + %(nf)d) + fmr_rt::TOTALS_BLOCKS;
+%(totals)s
+  FMR_CHECK(hipMemcpyAsync(out, res, %(nf)d * sizeof(double), hipMemcpyDeviceToHost, g_rt.stream));
+  FMR_CHECK(hipStreamSynchronize(g_rt.stream));
+}
+
+extern "C" long long Formura_HIP_Positivity_Violations(void) { return fmr_rt::violations(g_rt); }
+""" % dict(nf=len(self.fields), totals="\n".join(
+    "  fmr_rt::field_total(g_rt, FLD%d(g_rt.cur), %s, res + %d);" % (i, "*".join("%dLL" % n for n in self.N), i)
+    for i in range(len(self.fields))), advance=self.schedule.advance(self), forward_body=self.forward_body(shift, adv_filter),
            navi_init=navi_init, tmp_alloc=tmp_alloc, tmp_free=tmp_free, first=first, inst=inst))
         return "\n".join(L)
 
@@ -442,7 +466,7 @@ class Tile2DSchedule:
             def store(fname, v):
                 return "o%d = %s;" % (em.field_index[fname], v)
 
-            b = Body(node, load, lambda *a: None, lambda ax: None, store, "s%d_" % k)
+            b = Body(node, load, lambda *a: None, lambda ax: None, store, "s%d_" % k, prec=em.prec_of(node))
             body_lines.append("{")
             body_lines += ["  " + l for l in b.lines]
             body_lines.append("}")
@@ -551,7 +575,7 @@ class Stream3DSchedule:
             def store(fname, v):
                 return "o%d = %s;" % (em.field_index[fname], v)
 
-            b = Body(node, load, lambda *a: None, lambda ax: None, store, "s%d_" % k)
+            b = Body(node, load, lambda *a: None, lambda ax: None, store, "s%d_" % k, prec=em.prec_of(node))
             body_lines.append("{")
             body_lines += ["  " + l for l in b.lines]
             body_lines.append("}")

@@ -32,6 +32,7 @@ class Micro:
                      # ("Store", name, a)
     typ: str         # element type name (micro types are element types only)
     src: str = None  # source name of the OM node (set for literals bound to a name)
+    prec: str = None  # C type of the value in the mixed-precision form ("float"/"double")
 
 
 @dataclasses.dataclass
@@ -173,13 +174,19 @@ def gen_mm_program(om: OMProgram) -> MMProgram:
     )
 
 
-def inline_temporaries(mm: Dict[int, MMNode]) -> Dict[int, MMNode]:
+def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = None) -> Dict[int, MMNode]:
     """Replace every ``LoadCursor(c, X)`` of the Store nodes by X's own micro DAG
     with its cursors shifted by ``c`` (recursively), sharing identical
     instructions.  A manifest temporary holds, at each input position, the
     value of its DAG at that position (Templates.hs:442-474 index it so), so
     recomputing it where it is read gives the same numbers; the Store nodes of
     the result read only the state fields.  Returns the Store nodes only.
+
+    ``prec`` (node -> micro -> C type, from emit.mixed_precision on the nodes
+    before inlining) is carried into the copies and is part of the sharing key,
+    so a value keeps the precision of the expression it came from wherever it
+    is recomputed: the face flux that cell i recomputes as its left face is
+    the same FP32 computation as the one cell i-1 does as its right face.
     """
     out = {}
     for k in sorted(mm):
@@ -189,11 +196,11 @@ def inline_temporaries(mm: Dict[int, MMNode]) -> Dict[int, MMNode]:
         new: List[Micro] = []
         memo: Dict[tuple, int] = {}
 
-        def emit(inst, typ):
-            key = (inst, typ)
+        def emit(inst, typ, p=None):
+            key = (inst, typ, p)
             if key not in memo:
                 memo[key] = len(new)
-                new.append(Micro(inst, typ))
+                new.append(Micro(inst, typ, prec=p))
             return memo[key]
 
         def copy(src: MMNode, shift, j, cache):
@@ -201,30 +208,32 @@ def inline_temporaries(mm: Dict[int, MMNode]) -> Dict[int, MMNode]:
                 return cache[j]
             m = src.micro[j]
             k0 = m.inst[0]
+            p = prec[src.omid].get(j) if prec and src.omid in prec else None
+            emit_ = lambda inst, typ: emit(inst, typ, p)
             if k0 == "LoadCursorStatic":
-                r = emit(("LoadCursorStatic", _add(m.inst[1], shift), m.inst[2]), m.typ)
+                r = emit_(("LoadCursorStatic", _add(m.inst[1], shift), m.inst[2]), m.typ)
             elif k0 == "LoadCursor":
                 tgt = mm[m.inst[2]]
                 r = copy(tgt, _add(m.inst[1], shift), len(tgt.micro) - 1, {})
             elif k0 == "Imm" or k0 == "LoadIndex":
-                r = emit(m.inst, m.typ)
+                r = emit_(m.inst, m.typ)
                 if k0 == "Imm" and m.src:
                     new[r].src = m.src
             elif k0 == "Uniop":
-                r = emit(("Uniop", m.inst[1], copy(src, shift, m.inst[2], cache)), m.typ)
+                r = emit_(("Uniop", m.inst[1], copy(src, shift, m.inst[2], cache)), m.typ)
                 new[r].src = new[r].src or m.src
             elif k0 == "Binop":
                 a = copy(src, shift, m.inst[2], cache)
                 b = copy(src, shift, m.inst[3], cache)
-                r = emit(("Binop", m.inst[1], a, b), m.typ)
+                r = emit_(("Binop", m.inst[1], a, b), m.typ)
                 new[r].src = new[r].src or m.src
             elif k0 == "Triop":
                 a = copy(src, shift, m.inst[2], cache)
                 b = copy(src, shift, m.inst[3], cache)
                 c = copy(src, shift, m.inst[4], cache)
-                r = emit(("Triop", m.inst[1], a, b, c), m.typ)
+                r = emit_(("Triop", m.inst[1], a, b, c), m.typ)
             elif k0 == "Store":
-                r = emit(("Store", m.inst[1], copy(src, shift, m.inst[2], cache)), m.typ)
+                r = emit_(("Store", m.inst[1], copy(src, shift, m.inst[2], cache)), m.typ)
             else:
                 raise ValueError(k0)
             cache[j] = r

@@ -157,6 +157,19 @@ class FormuraLib:
         self._need_hip()
         return int(self.lib.Formura_HIP_Last_Launches())
 
+    def totals(self) -> dict:
+        """FP64 totals of the device state per field (fixed-order device sums)."""
+        self._need_hip()
+        out = (ctypes.c_double * len(self.fields))()
+        self.lib.Formura_HIP_Totals(out)
+        return {f: float(out[i]) for i, f in enumerate(self.fields)}
+
+    def positivity_violations(self) -> int:
+        """FP64 square-root arguments found not > 0 (mixed-precision libraries)."""
+        self._need_hip()
+        self.lib.Formura_HIP_Positivity_Violations.restype = ctypes.c_longlong
+        return int(self.lib.Formura_HIP_Positivity_Violations())
+
 
 def open_program(name: str) -> FormuraLib:
     """Open the HIP library of a bundled program (see ``formura_amd.build.PROGRAMS``)."""

@@ -44,6 +44,13 @@ void *Formura_HIP_State_Device(void);
 void *Formura_HIP_Stream(void);
 /* Kernel launches issued by the last Formura_Forward_Steps call. */
 long long Formura_HIP_Last_Launches(void);
+/* FP64 totals of every state field over the grid (out[f], fields in the order
+ * of formura_data), summed on the device in a fixed order (deterministic). */
+void Formura_HIP_Totals(double *out);
+/* Mixed-precision libraries: FP64 square-root arguments (wave speeds from
+ * pressure and density) found not > 0 since the library was loaded; 0 for
+ * libraries without such checks. */
+long long Formura_HIP_Positivity_Violations(void);
 
 #ifdef __cplusplus
 }

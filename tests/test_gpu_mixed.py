@@ -1,0 +1,107 @@
+"""FP64 totals of the device state, and the mixed-precision Euler form.
+
+* Formura_HIP_Totals equals the host's exactly rounded sum of the state
+  (math.fsum) to 1e-13;
+* the heat step conserves the total of q and the Euler step the five
+  conserved totals (flux differences telescope on a periodic grid), to
+  rounding -- also in the mixed form, whose update chain is FP64;
+* mixed Euler (FP32 flux arithmetic; FP64 wave speeds, positivity checks and
+  update) stays within 1e-5 relative max-norm of the FP64 oracle per field, with
+  no positivity violation, and its flat and LDS-streamed schedules agree bit
+  for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import hip_lib, layout_of, oracle_lib
+from formura_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _seeded(name, state):
+    h = hip_lib(name)
+    h.init()
+    h.set_state(state(h))
+    h.set_resident(True)
+    h.upload()
+    return h
+
+
+def _gs(h):
+    return inputs.grayscott_perturb(h.field("u").copy(), h.field("v").copy(), 3)
+
+
+def _heat(h):
+    return {"q": inputs.uniform(4, h.grid)}
+
+
+def _euler(h):
+    return inputs.euler_state(1, h.grid)
+
+
+@pytest.mark.parametrize("name,state", [("gs_64", _gs), ("heat3d_32", _heat), ("euler3d_16", _euler)])
+def test_totals_equal_host_sum(name, state):
+    h = _seeded(name, state)
+    tot = h.totals()
+    h.download()
+    for f in h.fields:
+        want = math.fsum(h.field(f).ravel().tolist())
+        assert abs(tot[f] - want) <= 1e-13 * max(abs(want), 1.0), (f, tot[f], want)
+    h.set_resident(False)
+    h.finalize()
+
+
+@pytest.mark.parametrize("name,state,steps,tol", [
+    ("heat3d_64", _heat, 100, 1e-13),
+    ("euler3d_32_inline_stream", _euler, 20, 1e-13),
+    ("euler3d_32_mixed_stream", _euler, 20, 1e-13),
+])
+def test_totals_conserved(name, state, steps, tol):
+    h = _seeded(name, state)
+    t0 = h.totals()
+    h.forward_steps(steps)
+    t1 = h.totals()
+    scale = max(abs(v) for v in t0.values())
+    for f in h.fields:
+        assert abs(t1[f] - t0[f]) <= tol * scale, (f, t0[f], t1[f])
+    h.set_resident(False)
+    h.finalize()
+
+
+def test_mixed_euler_close_to_fp64_oracle():
+    steps = 40
+    o = oracle_lib("euler3d_32", layout_of("euler3d_32"))
+    o.init()
+    st = inputs.euler_state(1, o.grid)
+    o.set_state(st)
+    o.forward(steps)
+    h = _seeded("euler3d_32_mixed_stream", lambda h: st)
+    h.forward_steps(steps)
+    h.download()
+    assert h.positivity_violations() == 0
+    errs = {}
+    for f in o.fields:
+        a, b = o.field(f), h.field(f)
+        errs[f] = float(np.max(np.abs(a - b)) / np.max(np.abs(a)))
+    print("mixed Euler 32^3, %d steps: relative max-norm error vs FP64 per field %s" % (steps, errs))
+    assert max(errs.values()) <= 1e-5, errs
+    assert max(errs.values()) > 0  # the FP32 flux arithmetic is really in use
+    h.set_resident(False)
+    h.finalize()
+
+
+def test_mixed_flat_and_streamed_agree():
+    st = None
+    out = []
+    for name in ("euler3d_32_mixed", "euler3d_32_mixed_stream"):
+        h = _seeded(name, lambda h: inputs.euler_state(2, h.grid))
+        h.forward_steps(9)
+        h.download()
+        out.append({f: h.field(f).copy() for f in h.fields})
+        h.set_resident(False)
+        h.finalize()
+    for f in out[0]:
+        np.testing.assert_array_equal(out[0][f], out[1][f])
