@@ -340,6 +340,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the heat 512^3, Euler 256^3 and adjoint legs")
+    ap.add_argument("--legs", default="plain,ensemble,heat,euler,adjoint,cpu",
+                    help="comma-separated legs to run (for profiling one kernel family): plain (Gray-Scott "
+                         "2048^2, the headline value), ensemble, heat, euler, adjoint, cpu")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB, 20 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -365,6 +368,23 @@ def main():
         print("bench.py: this build runs on one GPU only (the periodic halo is wrapped in HBM; "
               "no inter-GPU exchange is built). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    legs = set(args.legs.split(","))
+    if args.no_extras:
+        legs -= {"heat", "euler", "adjoint"}
+    if args.no_cpu:
+        legs.discard("cpu")
+    if args.ensemble <= 0:
+        legs.discard("ensemble")
+    if "plain" not in legs:
+        line = {"legs": sorted(legs)}
+        for key, leg, fn in (("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
+                             ("adjoint", "adjoint", run_adjoint)):
+            if leg in legs:
+                line[key] = fn()
+        if "ensemble" in legs:
+            line["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)
+        print(json.dumps(line), flush=True)
+        return
     name = "gs_2048"
     r = run_gpu(name, args.steps, args.warmup, 32.0)
     achieved = r["bytes_per_launch"] / (r["mean_launch_us"] * 1e-6) / 1e9
@@ -391,7 +411,7 @@ def main():
     }
     if not r["finite"]:
         line["error"] = "non-finite state after the run"
-    if args.ensemble > 0:
+    if "ensemble" in legs:
         try:
             ens = run_ensemble(args.ensemble, args.ensemble_steps)
             from formura_amd import batch as B
@@ -403,13 +423,14 @@ def main():
             line["ensemble"] = ens
         except Exception as e:
             line["ensemble"] = {"error": str(e)[:300]}
-    if not args.no_extras:
-        for key, fn in (("heat3d_512", run_heat_roofline), ("euler3d_256", run_euler), ("adjoint", run_adjoint)):
+    for key, leg, fn in (("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
+                         ("adjoint", "adjoint", run_adjoint)):
+        if leg in legs:
             try:
                 line[key] = fn()
             except Exception as e:
                 line[key] = {"error": str(e)[:300]}
-    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+    if "cpu" in legs and int(os.environ.get("RANK", "0")) == 0:
         try:
             line["cpu_baseline"] = cpu_baseline(name, args.cpu_seconds)
         except Exception as e:  # the GPU number stands without it
