@@ -572,8 +572,13 @@ __global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__
     if (USES_STATE && m > 0) xfetch(m - 1);
 %(sd)s
     const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
-    for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
-      const int r = idx / W, c = idx - r * W;
+    for (int idx = threadIdx.x; idx < HH * LW; idx += 256) {
+      // whole rows of the adjoint buffers, columns beyond the level masked: a
+      // wavefront's LDS accesses are contiguous (no bank conflicts at row wraps);
+      // measured faster here, while the forward kernels, bound by VALU, lose more
+      // to the masked lanes than they gain
+      const int r = idx / LW, c = idx - r * LW;
+      if (c >= W) continue;
       const bool own = r < TX && c < TY;
       %(adecl)s
       adj_point<XW, LW>(%(xargs)s, %(sargs)s, (r + 2 * SLEEVE) * XW + (c + 2 * SLEEVE), r * LW + c, own, P, %(aargs)s);
