@@ -150,6 +150,65 @@ def recompute_steps(nsteps: int, s: int, unit: int = 1) -> int:
     return total
 
 
+def reversal_ops(nsteps: int, unit: int, snapshots: int):
+    """The reversal as a list of operations in issue order, over numbered state
+    buffers: 0 holds the initial state, 1..len(first_descent) the states the
+    forward sweep kept, the others are work buffers.  (0, a, b, k): buffer b =
+    k forward steps from buffer a; (1, a, -1, k): the adjoint of the k steps
+    that start at buffer a.  Returns (ops, number of buffers).  Work buffers are
+    reused oldest-freed first, so forward work can run ahead of the adjoint."""
+    import collections
+    n = int(nsteps)
+    nunits = -(-n // unit) if n > 0 else 0
+    path = first_descent(nunits, snapshots)
+    kept_ids = list(range(1, 1 + len(path)))
+    ops = []
+    first_work = 1 + len(kept_ids)
+    free = collections.deque([first_work, first_work + 1])
+    count = [first_work + 2]
+
+    def alloc():
+        if free:
+            return free.popleft()
+        count[0] += 1
+        return count[0] - 1
+
+    def advance(src, pos, j):
+        dst = alloc()
+        ops.append((0, src, dst, unit_steps(n, unit, pos, pos + j)))
+        return dst
+
+    def adjoint(state, pos):
+        ops.append((1, state, -1, unit_steps(n, unit, pos, pos + 1)))
+
+    def reverse(base, pos, l, s, kept):
+        while True:
+            if l == 1:
+                adjoint(base, pos)
+                return
+            if s == 0:
+                for j in range(l - 1, -1, -1):
+                    if j:
+                        t = advance(base, pos, j)
+                        adjoint(t, pos + j)
+                        free.append(t)
+                    else:
+                        adjoint(base, pos)
+                return
+            j = split(l, s)
+            owned = not kept
+            mid = kept.pop(0) if kept else advance(base, pos, j)
+            reverse(mid, pos + j, l - j, s - 1, kept)
+            kept = []
+            if owned:
+                free.append(mid)
+            l = j
+
+    if nunits > 0:
+        reverse(0, 0, nunits, snapshots, kept_ids)
+    return ops, count[0]
+
+
 class Stepper:
     """One adjoint library: forward/adjoint steps over torch device tensors."""
 
@@ -166,6 +225,11 @@ class Stepper:
         L.Formura_AD_Forward.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.Formura_AD_Adjoint.argtypes = [ctypes.c_void_p] * 6
         L.Formura_AD_Adjoint_Steps.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int] + [ctypes.c_void_p] * 3
+        L.Formura_AD_Inner_States.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 2
+        L.Formura_AD_Adjoint_Levels.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int] + [ctypes.c_void_p] * 3
+        L.Formura_AD_Reverse.restype = ctypes.c_int
+        L.Formura_AD_Reverse.argtypes = ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+                                         + [ctypes.c_void_p] * 8)
         self.dtype = torch.float64 if self.desc["real"] == "double" else torch.float32
         self.shape = [len(self.desc["fields"])] + list(self.desc["grid"])
         self.nparams = len(self.desc["params"])
@@ -211,13 +275,56 @@ class Stepper:
                                           pgrad.data_ptr(), self._stream())
 
 
+    def _check_scratch(self, scratch, nlevels, like):
+        if scratch.dtype != self.dtype or scratch.numel() < nlevels * like.numel() or not scratch.is_contiguous():
+            raise ValueError("scratch holds fewer than %d states" % nlevels)
+
+    def inner_states(self, state0, scratch, nlevels, params_dev):
+        """The ``nlevels`` states after ``state0`` into ``scratch`` (first half of adjoint_steps)."""
+        self._check(state0)
+        if not 0 <= nlevels <= self.levels - 1:
+            raise ValueError("inner_states computes 0..%d levels, got %d" % (self.levels - 1, nlevels))
+        self._check_scratch(scratch, nlevels, state0)
+        self.lib.Formura_AD_Inner_States(state0.data_ptr(), scratch.data_ptr(), int(nlevels), params_dev.data_ptr(),
+                                         self._stream())
+
+    def adjoint_levels(self, state0, scratch, lam_next, lam_prev, nsteps, params_dev, pgrad):
+        """Second half of adjoint_steps: the adjoint over inner states already in ``scratch``."""
+        for t in (state0, lam_next, lam_prev):
+            self._check(t)
+        if not 1 <= nsteps <= self.levels:
+            raise ValueError("adjoint_levels reverses 1..%d steps, got %d" % (self.levels, nsteps))
+        self._check_scratch(scratch, nsteps - 1, state0)
+        self.lib.Formura_AD_Adjoint_Levels(state0.data_ptr(), scratch.data_ptr(), lam_next.data_ptr(),
+                                           lam_prev.data_ptr(), int(nsteps), params_dev.data_ptr(),
+                                           pgrad.data_ptr(), self._stream())
+
+
+    def reverse(self, ops, bufs, tmp, scratch, lam, lam2, params_dev, pgrad, stream_f=None):
+        """Run a ``reversal_ops`` schedule natively: forward work on ``stream_f``
+        (None: torch's current stream, i.e. one stream), adjoint work on torch's
+        current stream.  Returns the buffer (lam or lam2) holding the result."""
+        for t in list(bufs) + [tmp, lam, lam2]:
+            self._check(t)
+        if self.levels > 1:
+            self._check_scratch(scratch, 2 * (self.levels - 1), lam)
+        flat = (ctypes.c_int * (4 * len(ops)))(*[int(v) for op in ops for v in op])
+        ptrs = (ctypes.c_void_p * len(bufs))(*[t.data_ptr() for t in bufs])
+        a = self._stream()
+        f = ctypes.c_void_p(stream_f.cuda_stream) if stream_f is not None else a
+        which = self.lib.Formura_AD_Reverse(flat, len(ops), ptrs, len(bufs), tmp.data_ptr(), scratch.data_ptr(),
+                                            lam.data_ptr(), lam2.data_ptr(), params_dev.data_ptr(),
+                                            pgrad.data_ptr(), a, f)
+        return lam2 if which else lam
+
+
 class _Run(torch.autograd.Function):
     """The schedule runs over units of ``stepper.levels`` steps (the fused
     adjoint reverses one unit per call): checkpoints sit at unit boundaries and
     ``snapshots`` counts them."""
 
     @staticmethod
-    def forward(ctx, state0, params, stepper, nsteps, snapshots):
+    def forward(ctx, state0, params, stepper, nsteps, snapshots, overlap=True):
         st = stepper
         n, unit = int(nsteps), max(1, getattr(st, "levels", 1))
         p = params.detach().to(device=state0.device, dtype=torch.float64).contiguous()
@@ -238,6 +345,7 @@ class _Run(torch.autograd.Function):
         ctx.save_for_backward(x0, p)
         ctx.kept = kept
         ctx.stepper, ctx.nsteps, ctx.snapshots = st, n, int(snapshots)
+        ctx.overlap = overlap if overlap in ("one", "two", "python") else ("one" if overlap else "python")
         ctx.param_device, ctx.param_dtype = params.device, params.dtype
         return out
 
@@ -249,49 +357,7 @@ class _Run(torch.autograd.Function):
         lam = grad_out.detach().to(st.dtype).contiguous().clone()
         lam2 = torch.empty_like(lam)
         pgrad = torch.zeros(max(1, st.slots), max(1, st.nparams), dtype=torch.float64, device=x0.device)
-        tmp = torch.empty_like(x0)
-        scratch = st.scratch(x0.device) if unit > 1 else None
         stats = {"forward_steps": 0, "adjoint_steps": 0, "inner_forward_steps": 0, "adjoint_calls": 0}
-
-        def advance(src, pos, j):
-            k = unit_steps(n, unit, pos, pos + j)
-            dst = torch.empty_like(src)
-            st.forward(src, dst, tmp, k, p)
-            stats["forward_steps"] += k
-            return dst
-
-        def adjoint_unit(state, pos):
-            nonlocal lam, lam2
-            k = unit_steps(n, unit, pos, pos + 1)
-            if unit == 1:
-                st.adjoint(state, lam, lam2, p, pgrad)
-            else:
-                st.adjoint_steps(state, scratch, lam, lam2, k, p, pgrad)
-            lam, lam2 = lam2, lam
-            stats["adjoint_steps"] += k
-            stats["inner_forward_steps"] += k - 1
-            stats["adjoint_calls"] += 1
-
-        def reverse(base, pos, l, s, kept):
-            # base: state at unit `pos`, the start of the segment; reverse its l
-            # units with s free snapshots.  `kept` holds the states the forward
-            # sweep kept along this segment's first path.  The tail recursion over
-            # the head of the segment is a loop, so the depth is at most s.
-            while True:
-                if l == 1:
-                    adjoint_unit(base, pos)
-                    return
-                if s == 0:
-                    for j in range(l - 1, -1, -1):
-                        adjoint_unit(advance(base, pos, j) if j else base, pos + j)
-                    return
-                j = split(l, s)
-                mid = kept.pop(0) if kept else advance(base, pos, j)
-                reverse(mid, pos + j, l - j, s - 1, kept)
-                kept = []
-                del mid
-                l = j
-
         if getattr(st, "state_free", False):
             # the adjoint of a state-independent linear step needs no forward state
             for _ in range(n):
@@ -300,14 +366,50 @@ class _Run(torch.autograd.Function):
                 stats["adjoint_steps"] += 1
                 stats["adjoint_calls"] += 1
         elif n > 0:
+            ops, nbufs = reversal_ops(n, unit, ctx.snapshots)
             kept = list(ctx.kept)
             ctx.kept = None
-            reverse(x0, 0, -(-n // unit), ctx.snapshots, kept)
+            work = [torch.empty_like(x0) for _ in range(nbufs - 1 - len(kept))]
+            bufs = [x0] + kept + work
+            tmp = torch.empty_like(x0)
+            for kind, a, b, k in ops:
+                stats["forward_steps" if kind == 0 else "adjoint_steps"] += k
+                if kind == 1:
+                    stats["inner_forward_steps"] += k - 1
+                    stats["adjoint_calls"] += 1
+            native = ctx.overlap in ("one", "two") and hasattr(st, "reverse") and x0.is_cuda
+            if native:
+                # the library's executor; "two": forward work on a second stream
+                scratch = torch.empty([max(1, 2 * (unit - 1))] + list(x0.shape), dtype=x0.dtype, device=x0.device)
+                sf = torch.cuda.Stream(device=x0.device) if ctx.overlap == "two" else None
+                lam = st.reverse(ops, bufs, tmp, scratch, lam, lam2, p, pgrad, stream_f=sf)
+            else:
+                lam = _execute(st, ops, bufs, tmp, unit, lam, lam2, p, pgrad)
         ctx.stepper.last_stats = stats
         gp = pgrad.sum(0)[: st.nparams].to(device=ctx.param_device, dtype=ctx.param_dtype) if st.nparams else None
-        return lam.to(grad_out.dtype), (gp if gp is not None else None), None, None, None
+        return lam.to(grad_out.dtype), (gp if gp is not None else None), None, None, None, None
 
 
-def run(stepper: Stepper, state0: torch.Tensor, params: torch.Tensor, nsteps: int, snapshots: int = 32):
-    """State after ``nsteps`` steps, differentiable in ``state0`` and ``params``."""
-    return _Run.apply(state0, params, stepper, nsteps, snapshots)
+def _execute(st, ops, bufs, tmp, unit, lam, lam2, p, pgrad):
+    """Reference executor of a ``reversal_ops`` schedule, one call per operation
+    on the current stream (any stepper)."""
+    scratch = st.scratch(bufs[0].device) if unit > 1 else None
+    for kind, a, b, k in ops:
+        if kind == 0:
+            st.forward(bufs[a], bufs[b], tmp, k, p)
+        else:
+            if unit == 1:
+                st.adjoint(bufs[a], lam, lam2, p, pgrad)
+            else:
+                st.adjoint_steps(bufs[a], scratch, lam, lam2, k, p, pgrad)
+            lam, lam2 = lam2, lam
+    return lam
+
+
+def run(stepper: Stepper, state0: torch.Tensor, params: torch.Tensor, nsteps: int, snapshots: int = 32,
+        overlap="one"):
+    """State after ``nsteps`` steps, differentiable in ``state0`` and ``params``.
+    ``overlap``: "one" runs the reversal in the library's executor on one
+    stream, "two" with its forward work on a second stream, "python" (or False)
+    with one library call per operation."""
+    return _Run.apply(state0, params, stepper, nsteps, snapshots, overlap)

@@ -359,6 +359,9 @@ class AdjointEmitter:
             steps_call = ("  if (nsteps > 1) fwd_levels(nsteps - 1, (const char*)state0, (char*)scratch, params_dev, s);\n"
                           "  adj_levels(nsteps, (const char*)state0, (const char*)scratch, (const char*)lam_next, "
                           "(char*)lam_prev, params_dev, pgrad, s);")
+            inner_call = "  if (nlevels > 0) fwd_levels(nlevels, (const char*)state0, (char*)scratch, params_dev, s);"
+            levels_call = ("  adj_levels(nsteps, (const char*)state0, (const char*)scratch, (const char*)lam_next, "
+                           "(char*)lam_prev, params_dev, pgrad, s);")
             slots = nt
             levels = self.levels
         else:
@@ -371,6 +374,8 @@ class AdjointEmitter:
             steps_call = ("  (void)scratch;\n"
                           "  hipLaunchKernelGGL(k_adjoint, dim3(nb), dim3(BLOCK), 0, s, (const char*)state0, "
                           "(const char*)lam_next, (char*)lam_prev, params_dev, pgrad);")
+            inner_call = "  (void)state0; (void)scratch; (void)params_dev; (void)s;"
+            levels_call = steps_call
             slots = None
             levels = 1
         desc["levels"] = levels
@@ -380,6 +385,7 @@ class AdjointEmitter:
             ntot="*".join("(long long)%d" % n for n in self.N), flds=flds, lams=lams,
             desc=json.dumps(desc, sort_keys=True), fwd="\n".join("  " + l for l in fwd), adj="\n".join("  " + l for l in body),
             tiled=tiled_src, fwd_call=fwd_call, adj_call=adj_call, steps_call=steps_call, levels=levels,
+            inner_call=inner_call, levels_call=levels_call,
             slots=("%d" % slots) if slots else "(int)((NTOT + BLOCK - 1) / BLOCK)")
 
     def _emit_tiles(self, tile, uses_state):
@@ -635,6 +641,7 @@ static void adj_levels(int k, const char* st0, const char* lv, const char* lam_n
 ADJ_TEMPLATE = r'''// Generated by formura_amd: forward and adjoint steps of %(name)s. Do not edit.
 #include "formura_hip_rt.h"
 #include "formura_adjoint.h"
+#include <vector>
 
 namespace {
 typedef %(R)s real_t;
@@ -728,5 +735,98 @@ extern "C" void Formura_AD_Adjoint_Steps(const void* state0, void* scratch, cons
   if (nsteps < 1 || nsteps > KA) fmr_rt::fatal("Formura_AD_Adjoint_Steps: step count out of range");
 %(steps_call)s
   FMR_CHECK(hipGetLastError());
+}
+
+// The two halves of Formura_AD_Adjoint_Steps, for callers that overlap the
+// forward work of one unit with the adjoint work of another on two streams:
+// the nlevels (0..Formura_AD_Levels()-1) steps after state0, level i into
+// state buffer i of scratch ...
+extern "C" void Formura_AD_Inner_States(const void* state0, void* scratch, int nlevels, const double* params_dev,
+                                        void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (nlevels < 0 || nlevels > KA - 1) fmr_rt::fatal("Formura_AD_Inner_States: level count out of range");
+%(inner_call)s
+  FMR_CHECK(hipGetLastError());
+}
+
+// Executor of a reversal schedule (see include/formura_adjoint.h): forward ops on
+// stream F, adjoint ops on stream A, ordered by events on the buffers they share.
+// Events are re-recorded in place (a wait takes the event as last recorded).
+namespace {
+struct EventSet {
+  std::vector<hipEvent_t> e;
+  explicit EventSet(size_t n) : e(n) {
+    for (auto& x : e) FMR_CHECK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+  }
+  ~EventSet() {
+    for (auto x : e) (void)hipEventDestroy(x);
+  }
+};
+}  // namespace
+
+// ... and the nsteps adjoint steps over them (scratch as filled above).
+extern "C" void Formura_AD_Adjoint_Levels(const void* state0, const void* scratch, const void* lam_next, void* lam_prev,
+                                          int nsteps, const double* params_dev, double* pgrad, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned nb = (unsigned)((NTOT + BLOCK - 1) / BLOCK);
+  (void)nb;
+  if (nsteps < 1 || nsteps > KA) fmr_rt::fatal("Formura_AD_Adjoint_Levels: step count out of range");
+%(levels_call)s
+  FMR_CHECK(hipGetLastError());
+}
+
+extern "C" int Formura_AD_Reverse(const int* ops, int nops, void* const* bufs, int nbufs, void* tmp, void* scratch,
+                                  void* lam, void* lam2, const double* params_dev, double* pgrad, void* stream_a,
+                                  void* stream_f) {
+  hipStream_t A = (hipStream_t)stream_a, F = (hipStream_t)stream_f;
+  const bool two = F != A;
+  // events: [0, nbufs) last adjoint read of a buffer; nbufs+h last use of scratch
+  // half h; nbufs+2+h inner states of half h ready; nbufs+4 stream joins
+  EventSet ev(two ? (size_t)nbufs + 5 : 0);
+  std::vector<char> was_read(nbufs, 0);
+  char half_used[2] = {0, 0};
+  char* lcur = (char*)lam;
+  char* lnxt = (char*)lam2;
+  char* sc[2] = {(char*)scratch, (char*)scratch + (size_t)(KA > 1 ? KA - 1 : 0) * STATE_BYTES};
+  int turn = 0;
+  if (two) {  // F starts after everything A has queued (the saved states)
+    FMR_CHECK(hipEventRecord(ev.e[nbufs + 4], A));
+    FMR_CHECK(hipStreamWaitEvent(F, ev.e[nbufs + 4], 0));
+  }
+  for (int i = 0; i < nops; ++i) {
+    const int kind = ops[4 * i], a = ops[4 * i + 1], b = ops[4 * i + 2], k = ops[4 * i + 3];
+    if (a < 0 || a >= nbufs || (kind == 0 && (b < 0 || b >= nbufs || b == a)) || k < 1)
+      fmr_rt::fatal("Formura_AD_Reverse: bad operation");
+    if (kind == 0) {  // bufs[b] = k forward steps from bufs[a], on F
+      if (two && was_read[b]) FMR_CHECK(hipStreamWaitEvent(F, ev.e[b], 0));
+      Formura_AD_Forward(bufs[a], bufs[b], tmp, k, params_dev, (void*)F);
+    } else {  // the adjoint of the k steps from bufs[a], on A (inner states on F)
+      if (k > KA) fmr_rt::fatal("Formura_AD_Reverse: unit longer than Formura_AD_Levels()");
+      const int h = KA > 1 ? (turn++ & 1) : 0;
+      if (two && half_used[h]) FMR_CHECK(hipStreamWaitEvent(F, ev.e[nbufs + h], 0));
+      if (k > 1) Formura_AD_Inner_States(bufs[a], sc[h], k - 1, params_dev, (void*)F);
+      if (two) {
+        FMR_CHECK(hipEventRecord(ev.e[nbufs + 2 + h], F));
+        FMR_CHECK(hipStreamWaitEvent(A, ev.e[nbufs + 2 + h], 0));
+      }
+      Formura_AD_Adjoint_Levels(bufs[a], sc[h], lcur, lnxt, k, params_dev, pgrad, (void*)A);
+      char* t = lcur;
+      lcur = lnxt;
+      lnxt = t;
+      if (two) {
+        FMR_CHECK(hipEventRecord(ev.e[a], A));
+        was_read[a] = 1;
+        FMR_CHECK(hipEventRecord(ev.e[nbufs + h], A));
+        half_used[h] = 1;
+      }
+    }
+  }
+  if (two) {  // A's later work sees everything F did; the events die on return
+    FMR_CHECK(hipEventRecord(ev.e[nbufs + 4], F));
+    FMR_CHECK(hipStreamWaitEvent(A, ev.e[nbufs + 4], 0));
+    FMR_CHECK(hipStreamSynchronize(A));
+  }
+  FMR_CHECK(hipGetLastError());
+  return lcur == (char*)lam ? 0 : 1;
 }
 '''

@@ -32,6 +32,37 @@ void Formura_AD_Forward(const void *in, void *out, void *tmp, int nsteps, const 
 void Formura_AD_Adjoint(const void *state_prev, const void *lam_next, void *lam_prev, const double *params_dev,
                         double *pgrad, void *stream);
 
+/* Most steps one Formura_AD_Adjoint_Steps call reverses (1 for libraries
+ * with one-point-per-work-item kernels). */
+int Formura_AD_Levels(void);
+/* nsteps (1..Formura_AD_Levels()) adjoint steps in one call from the state at
+ * their start: the inner states are recomputed into scratch (nsteps-1 state
+ * buffers); lam_prev = dL/d(state0); pgrad += dL/dparams of the nsteps steps. */
+void Formura_AD_Adjoint_Steps(const void *state0, void *scratch, const void *lam_next, void *lam_prev, int nsteps,
+                              const double *params_dev, double *pgrad, void *stream);
+/* The two halves of Formura_AD_Adjoint_Steps, for a caller that runs the
+ * forward work of one unit on one stream while another stream reverses the
+ * previous unit: the inner states (nlevels = nsteps-1) ... */
+void Formura_AD_Inner_States(const void *state0, void *scratch, int nlevels, const double *params_dev, void *stream);
+/* ... and the adjoint steps over them. */
+void Formura_AD_Adjoint_Levels(const void *state0, const void *scratch, const void *lam_next, void *lam_prev,
+                               int nsteps, const double *params_dev, double *pgrad, void *stream);
+
+/* Executes a reversal schedule: nops operations of 4 ints (kind, a, b, k):
+ *   kind 0: bufs[b] = k forward steps from bufs[a]                (stream_f)
+ *   kind 1: the adjoint of the k (<= Formura_AD_Levels()) steps that start
+ *           at bufs[a]: inner states on stream_f into one half of scratch
+ *           (2*(levels-1) state buffers, halves used in turn), adjoint steps
+ *           on stream_a from lam to lam2 (then the two swap)   (stream_a)
+ * Events order the two streams on every shared buffer, so the forward work of
+ * later operations overlaps the adjoint ones (stream_f == stream_a: one
+ * stream, no events).  lam holds dL/d(output) on entry; returns 0 if the
+ * result dL/d(bufs of the first adjoint's state...) ends in lam, 1 if in lam2.
+ * pgrad accumulates as in Formura_AD_Adjoint.  Returns after stream_a has
+ * finished when the two streams differ. */
+int Formura_AD_Reverse(const int *ops, int nops, void *const *bufs, int nbufs, void *tmp, void *scratch, void *lam,
+                       void *lam2, const double *params_dev, double *pgrad, void *stream_a, void *stream_f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,10 +51,10 @@ def _gs_problem(n=64, steps=30, seed=3):
     return s, x0, tgt[1].detach()
 
 
-def _gpu_loss_and_grads(s, x0, theta, vt, steps, snaps):
+def _gpu_loss_and_grads(s, x0, theta, vt, steps, snaps, overlap="one"):
     x = torch.tensor(x0, device=DEV, requires_grad=True)
     p = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
-    out = A.run(s, x, p, steps, snaps)
+    out = A.run(s, x, p, steps, snaps, overlap=overlap)
     loss = 0.5 * ((out[1] - vt) ** 2).sum()
     loss.backward()
     return loss.item(), x.grad.cpu().numpy(), p.grad.cpu().numpy()
@@ -121,6 +121,19 @@ def test_snapshot_count_does_not_change_gradients():
         np.testing.assert_array_equal(r[2], res[0][2])
     assert s.last_stats["forward_steps"] == A.recompute_steps(steps, 32, s.levels)
     assert s.last_stats["adjoint_steps"] == steps
+
+
+@pytest.mark.parametrize("mode", ["one", "two"])
+def test_native_reversal_matches_python_executor(mode):
+    """The library's executor of the schedule (one stream, or the forward work
+    on a second stream ordered by events) gives the same gradients bit for bit
+    as the reference executor (one library call per operation)."""
+    steps = 45
+    s, x0, vt = _gs_problem(steps=steps)
+    a = _gpu_loss_and_grads(s, x0, THETA, vt, steps, 3, overlap=mode)
+    b = _gpu_loss_and_grads(s, x0, THETA, vt, steps, 3, overlap="python")
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[2], b[2])
 
 
 def test_heat_adjoint_matches_torch_cpu_reference():
