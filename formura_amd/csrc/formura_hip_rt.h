@@ -40,6 +40,28 @@ __device__ __forceinline__ int fmr_mod(int i) {
   return r < 0 ? r + N : r;
 }
 
+// a / b for a divisor b fixed at generation time, correctly rounded (the value
+// of IEEE division) without the device's division sequence.  The generator
+// passes y = RN(1/b), computed on the host, and b > 0 with 2^-100 <= b <= 2^100.
+//   q0 = RN(a*y) is within 2 ulp of a/b;  q1 = q0 - (b*q0 - a)*y (fused) is
+//   faithful;  q2 = q1 - (b*q1 - a)*y (fused) is RN(a/b) by Markstein's theorem
+//   (IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of Floating-Point
+//   Arithmetic, 2nd ed., Thm. 4.10): b*q1 - a is exact for a faithful q1, and
+//   y has relative error below 2^-53.
+// The theorem needs no underflow or overflow: with 2^-900 <= |a| <= 2^900 the
+// quotient and the remainders are normal.  Other a (tiny, huge, Inf, NaN) take
+// the division.  The remainder is formed as b*q - a and subtracted, so a zero
+// dividend keeps its sign.
+__host__ __device__ __forceinline__ double fmr_divc(double a, double b, double y) {
+  const double aa = fabs(a);
+  if (__builtin_expect(!((aa >= 0x1p-900 && aa <= 0x1p+900) || aa == 0.0), 0)) return a / b;
+  double q = a * y;
+  double r = fma(b, q, -a);
+  q = fma(-r, y, q);
+  r = fma(b, q, -a);
+  return fma(-r, y, q);
+}
+
 // Positivity checks of the mixed-precision form: every FP64 square-root
 // argument (e.g. gamma*p/rho of a wave speed) must be > 0; a non-positive or
 // NaN argument is counted (vector atomic, taken only on a violation).

@@ -171,6 +171,14 @@ def _fold(m, vals, typ):
     return None
 
 
+def _divc_ok(b) -> bool:
+    """A constant divisor that ``fmr_divc`` handles: finite, 2^-100 <= |b| <= 2^100."""
+    if isinstance(b, bool):
+        return False
+    b = float(b)
+    return math.isfinite(b) and 2.0 ** -100 <= abs(b) <= 2.0 ** 100
+
+
 def _const_literal(v, typ):
     if typ == "bool":
         return "true" if v else "false"
@@ -185,7 +193,7 @@ class Body:
     def __init__(self, node: MMNode, load: Callable, tmp_load: Callable, index: Callable,
                  store: Callable, prefix: str, fold: bool = True, const_kw="const ",
                  params: Dict[str, str] = None, real: str = None, recip: bool = False,
-                 prec: Dict[int, str] = None):
+                 prec: Dict[int, str] = None, divc: bool = True):
         """``params``: literal names bound to run-time expressions (not folded);
         ``real``: if set ("float"), every floating micro value is computed in
         that type (the FP32 form; constants are folded in double and rounded,
@@ -195,7 +203,12 @@ class Body:
         ``prec``: per-micro C type of the floating values ("double"/"float",
         from ``mixed_precision``); operands of another precision are converted,
         FP32 values divide by constants through the reciprocal, and every
-        FP64 square-root argument is checked to be positive (FMR_POSITIVE)."""
+        FP64 square-root argument is checked to be positive (FMR_POSITIVE);
+        ``divc``: an FP64 division by a constant is emitted as ``fmr_divc``
+        (device code: the correctly rounded quotient from the host-rounded
+        reciprocal and two fused corrections) or, for a power-of-two
+        divisor, as the exact product with its reciprocal; host code keeps
+        the division."""
         self.lines: List[str] = []
         vals: Dict[int, object] = {}
         params = params or {}
@@ -236,6 +249,22 @@ class Body:
                 if prec and ctypes.get(m.inst[2]) in ("float", "double") and ctypes[m.inst[2]] != ct:
                     src = "(%s)%s" % (ct, src)
                 self.lines.append("%s%s %s = %s*%s;" % (const_kw, ct, name, src, lit))
+                continue
+            if (divc and ct == "double" and k == "Binop" and m.inst[1] == "/" and m.inst[3] in vals
+                    and m.inst[2] not in vals and _divc_ok(vals[m.inst[3]])):
+                b = float(vals[m.inst[3]])
+                src = "%s%d" % (prefix, m.inst[2])
+                if prec and ctypes.get(m.inst[2]) == "float":
+                    src = "(double)%s" % src
+                mant, _ = math.frexp(abs(b))
+                if mant == 0.5:
+                    # 1/b is exact: a*(1/b) and a/b are the same real number, rounded once
+                    rhs = "%s*%s" % (src, show_double(1.0 / b))
+                elif b > 0:
+                    rhs = "fmr_divc(%s, %s, %s)" % (src, show_double(b), show_double(1.0 / b))
+                else:
+                    rhs = "-fmr_divc(%s, %s, %s)" % (src, show_double(-b), show_double(-1.0 / b))
+                self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, rhs))
                 continue
             if fold:
                 ops = _operands(m)

@@ -217,7 +217,7 @@ class LibEmitter:
                     return "formura_mn%d%s = %s;" % (k, "".join("[%s]" % i for i in ivs), v)
                 return "%s.%s%s = %s;" % (inst, fname, "".join("[%s]" % i for i in ivs), v)
 
-            body = Body(node, load, tmp_load, index, store, "a", fold=False)
+            body = Body(node, load, tmp_load, index, store, "a", fold=False, divc=False)
             lines = list(body.lines)
             if is_tmp:
                 lines.append(store(None, body.result))
