@@ -1,0 +1,47 @@
+// Host check of fmr_divc (formura_hip_rt.h): the division of a by a constant b
+// through y = RN(1/b) and two fused corrections must equal IEEE a/b bit for bit.
+// Cases: random dividends over the fast range, dividends whose quotient lies
+// next to a rounding midpoint (the hard cases of correct rounding), stencil-like
+// differences, and specials (signed zeros, subnormals, the range limits, Inf, NaN).
+#include "formura_hip_rt.h"
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+static uint64_t s = 0x9E3779B97F4A7C15ull;
+static inline uint64_t nx() { uint64_t z = (s += 0x9E3779B97F4A7C15ull); z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull; z = (z ^ (z >> 27)) * 0x94D049BB133111EBull; return z ^ (z >> 31); }
+static inline uint64_t bits(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
+static inline double dbl(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
+int main(int argc, char** argv) {
+  long n = argc > 1 ? atol(argv[1]) : 10000000;
+  if (argc > 2) s = strtoull(argv[2], nullptr, 0);
+  double bs[] = {1e-4, 0.01, 3.0, 7.0, 0.1, 1.0/3.0, 20.48, 1e-5, 1.7976931348623157e+30, 0x1.fffffffffffffp-1, 0x1.0000000000001p0, 9.999999999999999e-5};
+  long bad = 0, tot = 0;
+  for (double b : bs) {
+    double y = 1.0 / b;
+    for (long i = 0; i < n; ++i) {
+      uint64_t r = nx();
+      double a;
+      int kind = i % 3;
+      if (kind == 0) {  // random bits over the fast range
+        int e = (int)(r % 1800) - 900;
+        a = ldexp(1.0 + (double)(nx() >> 12) * 0x1p-52, e);
+        if (r & (1ull << 63)) a = -a;
+      } else if (kind == 1) {  // near midpoints: a = RN(b*m) with m a midpoint
+        double q = ldexp(1.0 + (double)(nx() >> 12) * 0x1p-52, (int)(r % 200) - 100);
+        double m = q + ldexp(1.0, ilogb(q) - 53);
+        a = b * m;
+        if (r & 1) a = nextafter(a, 0.0);
+        if (r & 2) a = nextafter(a, 1e308);
+      } else {  // values like stencil differences
+        a = ((double)(nx() >> 11) * 0x1p-53 - 0.5) * ldexp(1.0, (int)(r % 60) - 50);
+      }
+      double q1 = fmr_divc(a, b, y), q2 = a / b;
+      ++tot;
+      if (bits(q1) != bits(q2)) { if (bad < 10) printf("MISMATCH b=%a a=%a got %a want %a\n", b, a, q1, q2); ++bad; }
+    }
+    double sp[] = {0.0, -0.0, 1e-310, -1e-310, 1e308, -1e308, INFINITY, -INFINITY, NAN, 0x1p-900, 0x1p900, 0x1p-901, 0x1.fffffffffffffp900, 5e-324};
+    for (double a : sp) { double q1 = fmr_divc(a, b, y), q2 = a / b; ++tot; if (bits(q1) != bits(q2) && !(q1 != q1 && q2 != q2)) { printf("SPECIAL MISMATCH b=%a a=%a got %a want %a\n", b, a, q1, q2); ++bad; } }
+  }
+  printf("checked %ld, mismatches %ld\n", tot, bad);
+  return bad != 0;
+}

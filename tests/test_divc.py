@@ -1,0 +1,51 @@
+"""The device's division by a generation-time constant (``fmr_divc`` in
+``formura_amd/csrc/formura_hip_rt.h``) equals IEEE division bit for bit.
+
+The step kernels divide by constants (``/(dx*dx)`` in every bundled program)
+through a host-rounded reciprocal and two fused corrections; the oracle's C
+divides.  The same header is compiled for the host (x86 FMA is correctly
+rounded, as the device's is) and checked on random, near-midpoint and special
+dividends for several divisors.  On the GPU the parity tests then compare the
+kernels themselves with the oracle.
+"""
+import os
+import subprocess
+
+import pytest
+
+from formura_amd import build
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("divc") / "divc_check")
+    cmd = [build.HIPCC, "-O2", "-mfma", "-ffp-contract=off", "-I", build.CSRC,
+           os.path.join(HERE, "native", "divc_check.cpp"), "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("host build of the division check failed: %s" % r.stderr[-500:])
+    return out
+
+
+@pytest.mark.parametrize("seed", ["0x9E3779B97F4A7C15", "12345"])
+def test_divc_matches_ieee_division(checker, seed):
+    r = subprocess.run([checker, "2000000", seed], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "mismatches 0" in r.stdout
+
+
+def test_emitted_divisions():
+    """A non-power-of-two constant divisor becomes fmr_divc with its rounded
+    reciprocal; a power-of-two one an exact product; host Setup keeps '/'."""
+    from formura_amd.emit_hip import generate
+    _, src, _ = generate(os.path.join(build.PROG_DIR, "grayscott2d.fmr"),
+                         os.path.join(build.PROG_DIR, "gs_64.yaml"), "gs_64")
+    assert src.count(", 0.0001, 10000.0)") == 4  # d_xx and d_yy of u and of v
+    setup = src[src.index("static void Formura_Setup"):]
+    assert "fmr_divc" not in setup[:setup.index("\n}\n")]
+    _, hsrc, _ = generate(os.path.join(build.PROG_DIR, "heat3d.fmr"),
+                          os.path.join(build.PROG_DIR, "heat3d_16.yaml"), "heat3d_16")
+    kernels = hsrc[:hsrc.index("static void Formura_Setup")]
+    assert "fmr_divc" not in kernels and "*256.0" in kernels
