@@ -332,6 +332,38 @@ def cpu_baseline(name: str, target_s: float = 10.0):
                       "-mavx2 -mfma -fopenmp, OMP_NUM_THREADS=%d" % (n, name, el, threads)}
 
 
+def run_samples(steps: int = 50):
+    """The reference's sample programs at 128^3 (3-D HLLD MHD with GLM cleaning,
+    3-D HLLC with MUSCL-Hancock), compiled from their .fmr by this front-end
+    with the default schedule: GLUPS with the state resident in HBM, and the
+    FP64 totals' relative drift (every field but the damped psi)."""
+    import numpy as np
+    from formura_amd import build as B, inputs
+    from formura_amd.runtime import open_program
+    out = {"steps": steps}
+    for name, gamma in (("mhd3d_128", 5.0 / 3.0), ("hllc3d_128", 7.0 / 5.0)):
+        if not os.path.exists(B.lib_path(name)):
+            out[name] = {"skipped": "built only where the formura checkout is present"}
+            continue
+        lib = open_program(name)
+        lib.init()
+        lib.set_state(inputs.conserved_state(5, tuple(lib.grid), lib.fields, gamma))
+        lib.set_resident(True)
+        lib.upload()
+        lib.forward_steps(2)
+        t0 = lib.totals()
+        ms = lib.forward_steps(steps)
+        t1 = lib.totals()
+        cells = int(np.prod(lib.grid))
+        cons = [f for f in t0 if f != "psi"]
+        out[name] = {"glups": cells * steps / (ms * 1e-3) / 1e9, "ms": ms, "schedule": lib.layout.get("schedule"),
+                     "fields": len(lib.fields),
+                     "max_rel_total_drift": max(abs(t1[f] - t0[f]) for f in cons) / max(abs(t0[f]) for f in cons)}
+        lib.set_resident(False)
+        lib.finalize()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,9 +376,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the heat 512^3, Euler 256^3 and adjoint legs")
-    ap.add_argument("--legs", default="plain,ensemble,heat,euler,adjoint,cpu",
+    ap.add_argument("--legs", default="plain,ensemble,heat,euler,adjoint,samples,cpu",
                     help="comma-separated legs to run (for profiling one kernel family): plain (Gray-Scott "
-                         "2048^2, the headline value), ensemble, heat, euler, adjoint, cpu")
+                         "2048^2, the headline value), ensemble, heat, euler, adjoint, samples, cpu")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB, 20 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -374,7 +406,7 @@ def main():
         sys.exit(2)
     legs = set(args.legs.split(","))
     if args.no_extras:
-        legs -= {"heat", "euler", "adjoint"}
+        legs -= {"heat", "euler", "adjoint", "samples"}
     if args.no_cpu:
         legs.discard("cpu")
     if args.ensemble <= 0:
@@ -382,7 +414,7 @@ def main():
     if "plain" not in legs:
         line = {"legs": sorted(legs)}
         for key, leg, fn in (("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
-                             ("adjoint", "adjoint", run_adjoint)):
+                             ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples)):
             if leg in legs:
                 line[key] = fn()
         if "ensemble" in legs:
@@ -428,7 +460,7 @@ def main():
         except Exception as e:
             line["ensemble"] = {"error": str(e)[:300]}
     for key, leg, fn in (("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
-                         ("adjoint", "adjoint", run_adjoint)):
+                         ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples)):
         if leg in legs:
             try:
                 line[key] = fn()

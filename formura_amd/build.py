@@ -64,6 +64,34 @@ PROGRAMS: Dict[str, tuple] = {
 }
 
 
+# The reference's own sample programs (the .fmr files under sample/ of the
+# formura checkout), compiled by this front-end and emitter with one-rank
+# configs of this package: name -> (program path relative to the checkout,
+# config, schedule).  They are built only where the checkout is present
+# (FORMURA_REFERENCE, default /root/reference); the built libraries are then
+# used like any other.  Their .fmr text is not part of this package.
+REFERENCE_ROOT = os.environ.get("FORMURA_REFERENCE", "/root/reference")
+SAMPLES: Dict[str, tuple] = {
+    "hllc1d_64": ("sample/1d_hllc/hllc.fmr", "hllc1d_64.yaml", "auto"),
+    "hllc1d_400": ("sample/1d_hllc/hllc.fmr", "hllc1d_400.yaml", "auto"),
+    "mhd1d_64": ("sample/mhd_1d_hlld/mhd_hlld.fmr", "mhd1d_64.yaml", "auto"),
+    "mhd3d_8": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_8.yaml", "auto"),
+    "hllc3d_8": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_8.yaml", "auto"),
+    "mhd3d_128": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_128.yaml", "auto"),
+    "hllc3d_128": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_128.yaml", "auto"),
+}
+
+
+def reference_present() -> bool:
+    return os.path.isdir(os.path.join(REFERENCE_ROOT, "sample"))
+
+
+def build_sample(name: str, force: bool = False) -> str:
+    rel, yml, schedule = SAMPLES[name]
+    d = generate_program(name, os.path.join(REFERENCE_ROOT, rel), os.path.join(PROG_DIR, yml), schedule)
+    return compile_library(name, d, force=force)
+
+
 # batched ensembles: name -> (program, config, precision, lifted literals, tile x, tile y, steps per launch)
 BATCHES: Dict[str, tuple] = {
     "gs_batch_f64": ("grayscott2d.fmr", "gs_64.yaml", "f64", ["f", "k"], 16, 32, 3),
@@ -205,7 +233,8 @@ def build_one(name: str, force: bool = False) -> str:
 
 
 def build_all(names: Optional[List[str]] = None, force: bool = False, jobs: int = 0) -> List[str]:
-    names = names or (list(PROGRAMS) + list(BATCHES) + list(ADJOINTS) + [side_name(n) for n in ENSEMBLE_SIDES])
+    names = names or (list(PROGRAMS) + list(BATCHES) + list(ADJOINTS) + [side_name(n) for n in ENSEMBLE_SIDES]
+                      + (list(SAMPLES) if reference_present() else []))
     jobs = jobs or min(8, os.cpu_count() or 4)
     outs = []
 
@@ -216,6 +245,8 @@ def build_all(names: Optional[List[str]] = None, force: bool = False, jobs: int 
             return build_adjoint
         if n.startswith("gs_side_"):
             return build_side
+        if n in SAMPLES:
+            return build_sample
         return build_one
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -79,3 +79,36 @@ def euler_cfl_dt(seed: int, shape, dx: float, cfl: float = 0.4) -> float:
     rho, u, v, w, p = euler_primitive(seed, shape)
     c = np.sqrt(EULER_GAMMA * p / rho)
     return float(cfl * dx / np.max(np.abs(u) + np.abs(v) + np.abs(w) + 3.0 * c))
+
+
+def conserved_state(seed: int, shape, fields, gamma: float) -> dict:
+    """A seeded, physically valid conserved state for an Euler or MHD program,
+    keyed by the program's own field names: rho = 1 + 0.1 U[-1,1), p = 1 + 0.1
+    U[-1,1), velocity components 0.1 U[-1,1), magnetic components (B_x, B_y,
+    B_z) 0.3 U[-1,1), psi 0.01 U[-1,1); momenta rho*V (rhou, rhov, rhow), total
+    energy e = p/(gamma-1) + rho |V|^2 / 2 + |B|^2 / 2 (the B fields present).
+    Each quantity draws from its own SplitMix64 stream in row-major order."""
+    def draw(k):
+        return 2.0 * uniform(seed * 32 + k, shape) - 1.0
+    rho = 1.0 + 0.1 * draw(0)
+    p = 1.0 + 0.1 * draw(1)
+    vel = {c: 0.1 * draw(2 + i) for i, c in enumerate("uvw")}
+    B = {c: 0.3 * draw(5 + i) for i, c in enumerate("xyz") if "B_" + c in fields}
+    out = {}
+    kin = sum(v * v for v in vel.values())
+    mag = sum(b * b for b in B.values()) if B else 0.0
+    e = p / (gamma - 1.0) + 0.5 * rho * kin + 0.5 * mag
+    for f in fields:
+        if f == "rho":
+            out[f] = rho
+        elif f in ("rhou", "rhov", "rhow"):
+            out[f] = rho * vel[f[-1]]
+        elif f == "e":
+            out[f] = e
+        elif f.startswith("B_"):
+            out[f] = B[f[-1]]
+        elif f == "psi":
+            out[f] = 0.01 * draw(8)
+        else:
+            raise ValueError("conserved_state: unknown field %r" % f)
+    return out

@@ -19,10 +19,10 @@ def _shift(a, offs):
     return np.roll(a, shift=[-o for o in offs], axis=tuple(range(a.ndim)))
 
 
-def _eval_node(node, fields, temps, base, params=None):
+def _eval_node(node, fields, temps, base, params=None, shape=None, offsets=None):
     vals = []
     params = params or {}
-    ext = {"sqrt": np.sqrt, "exp": np.exp, "fabs": np.fabs}
+    ext = {"sqrt": np.sqrt, "exp": np.exp, "fabs": np.fabs, "sin": np.sin, "cos": np.cos}
     for m in node.micro:
         k = m.inst[0]
         if k == "Imm":
@@ -31,6 +31,11 @@ def _eval_node(node, fields, temps, base, params=None):
             v = _shift(fields[m.inst[2]], [c + b for c, b in zip(m.inst[1], base)])
         elif k == "LoadCursor":
             v = _shift(temps[m.inst[2]], [c + b for c, b in zip(m.inst[1], base)])
+        elif k == "LoadIndex":
+            # i_k + offset_k (Templates.hs:481, block offset 0), as a double
+            ax = m.inst[1]
+            ix = np.arange(shape[ax], dtype=np.float64) + float((offsets or [0] * len(shape))[ax])
+            v = ix.reshape([-1 if a == ax else 1 for a in range(len(shape))])
         elif k == "Uniop":
             a = vals[m.inst[2]]
             op = m.inst[1]
@@ -73,7 +78,23 @@ def step(mm_graph, sleeve, fields, params=None):
         node = mm_graph[k]
         is_store = node.micro[-1].inst[0] == "Store"
         base = [rng[k].offset - 2 * sleeve] * len(shape) if is_store else [0] * len(shape)
-        v = np.broadcast_to(np.asarray(_eval_node(node, fields, temps, base, params), dtype=np.float64), shape)
+        v = np.broadcast_to(np.asarray(_eval_node(node, fields, temps, base, params, shape), dtype=np.float64), shape)
+        if is_store:
+            out[node.micro[-1].inst[1]] = np.array(v)
+        else:
+            temps[k] = np.array(v)
+    return out
+
+
+def init(mm_init, shape):
+    """The init graph at offset 0 (Formura_Setup, Templates.hs:272-280): every node
+    over the whole grid, temporaries indexed by position, no halo."""
+    temps, out = {}, {}
+    for k in sorted(mm_init):
+        node = mm_init[k]
+        is_store = node.micro[-1].inst[0] == "Store"
+        v = np.broadcast_to(np.asarray(_eval_node(node, {}, temps, [0] * len(shape), None, shape),
+                                       dtype=np.float64), shape)
         if is_store:
             out[node.micro[-1].inst[1]] = np.array(v)
         else:

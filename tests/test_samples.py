@@ -1,0 +1,91 @@
+"""The reference's own sample programs through this front-end and HIP emitter.
+
+sample/1d_hllc (HLLC, first order), sample/mhd_1d_hlld and sample/mhd_3d_hlld
+(HLLD MHD, second order, GLM cleaning in 3-D) and sample/3d_hllc (HLLC with the
+MUSCL-Hancock reconstruction, the program's default) of formura v2.3.2 are
+compiled with one-rank configs of this package (``build.SAMPLES``).  The
+fixtures in tests/golden/sample_*.npz are the numpy interpreter's runs of the
+same graphs, checked when written for conservation, uniform-state invariance
+and, for the 400-cell shock tube from the program's own init graph, against
+the reference's exact Riemann solution (tests/golden/make_samples.py).
+
+CPU: the interpreter reproduces every fixture bit for bit (needs the formura
+checkout; skipped without it).  GPU: each built library, seeded the same way,
+gives the fixture bit for bit after the same number of Formura_Forward calls,
+with the same Navi offsets; the shock tube also sits within the fixture's
+distance of the exact solution.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_lib
+from formura_amd import build as B, inputs
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = ["hllc1d_64", "mhd1d_64", "hllc3d_8", "mhd3d_8", "hllc1d_400"]
+
+
+def fixture(name):
+    with open(os.path.join(GOLDEN, "sample_%s.json" % name)) as f:
+        meta = json.load(f)
+    with np.load(os.path.join(GOLDEN, "sample_%s.npz" % name)) as z:
+        out = {k: z[k] for k in z.files}
+    return meta, out
+
+
+def exact_interior_l1(rho_frame, steps):
+    ex = np.loadtxt(os.path.join(GOLDEN, "hllc1d_exact.txt"))
+    n = rho_frame.shape[0]
+    k = (np.arange(n) + (-2 * steps) % n) % n
+    rho = np.empty(n)
+    rho[k] = rho_frame
+    return float(np.mean(np.abs(rho - ex[:, 1])[150:350]))
+
+
+@pytest.mark.skipif(not B.reference_present(), reason="formura checkout not present")
+@pytest.mark.parametrize("name", CASES)
+def test_interpreter_reproduces_fixture(name):
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import make_samples as M
+    meta, want = fixture(name)
+    mm, desc = M.front_end(name)
+    state = M.initial_state(name, mm, desc)
+    got = M.run(mm, desc, state, meta["steps"])
+    assert sorted(got) == meta["fields"]
+    for f in got:
+        np.testing.assert_array_equal(got[f], want[f], err_msg=f)
+
+
+def test_shock_tube_fixture_near_exact_solution():
+    meta, out = fixture("hllc1d_400")
+    assert exact_interior_l1(out["rho"], meta["steps"]) < 0.01
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_sample_library_matches_fixture_bit_exact(name):
+    path = B.lib_path(name)
+    if not os.path.exists(path):
+        pytest.skip("%s is built only where the formura checkout is present" % name)
+    meta, want = fixture(name)
+    print("sample %s: %d steps, grid %s" % (name, meta["steps"], meta["grid"]), flush=True)
+    lib = load_lib(path)
+    lib.init()
+    if meta["seed"] is not None:
+        lib.set_state(inputs.conserved_state(meta["seed"], tuple(meta["grid"]), lib.fields, meta["gamma"]))
+    lib.forward(meta["steps"])
+    nv = lib.navi_dict()
+    assert nv["time_step"] == meta["steps"]
+    for ax, off in zip(lib.axes, meta["offset"]):
+        assert nv["offset_" + ax] == off
+    for f in lib.fields:
+        got = lib.field(f)
+        if not np.array_equal(got, want[f]):
+            raise AssertionError("%s field %s differs from the fixture: max |diff| %g"
+                                 % (name, f, float(np.max(np.abs(got - want[f])))))
+    if name == "hllc1d_400":
+        assert exact_interior_l1(lib.field("rho"), meta["steps"]) < 0.01
