@@ -40,6 +40,7 @@ PROGRAMS: Dict[str, tuple] = {
     "heat3d_256": ("heat3d.fmr", "heat3d_256.yaml", "auto"),
     "heat3d_512": ("heat3d.fmr", "heat3d_512.yaml", "auto"),
     "gs_64": ("grayscott2d.fmr", "gs_64.yaml", "auto"),
+    "hooks2d_32": ("hooks2d.fmr", "hooks2d_32.yaml", "auto"),
     "gs_1024": ("grayscott2d.fmr", "gs_1024.yaml", "auto"),
     "gs_2048": ("grayscott2d.fmr", "gs_2048.yaml", "auto"),
     "gs_8192": ("grayscott2d.fmr", "gs_8192.yaml", "auto"),
@@ -86,10 +87,36 @@ def reference_present() -> bool:
     return os.path.isdir(os.path.join(REFERENCE_ROOT, "sample"))
 
 
+# The reference's own driver of a sample, built unchanged against the generated
+# header (under the name the driver includes) and the HIP library:
+# name -> (driver path relative to the checkout, header name it includes).
+SAMPLE_DRIVERS: Dict[str, tuple] = {
+    "hllc1d_400": ("sample/1d_hllc/main.c", "hllc.h"),
+}
+
+
+def driver_path(name: str) -> str:
+    return os.path.join(BUILD_DIR, name, "driver", "main")
+
+
 def build_sample(name: str, force: bool = False) -> str:
     rel, yml, schedule = SAMPLES[name]
     d = generate_program(name, os.path.join(REFERENCE_ROOT, rel), os.path.join(PROG_DIR, yml), schedule)
-    return compile_library(name, d, force=force)
+    lib = compile_library(name, d, force=force)
+    if name in SAMPLE_DRIVERS:
+        src, hdr = SAMPLE_DRIVERS[name]
+        dd = os.path.join(d, "driver")
+        os.makedirs(dd, exist_ok=True)
+        with open(os.path.join(d, "%s.h" % name)) as f:
+            text = f.read()
+        with open(os.path.join(dd, hdr), "w") as f:
+            f.write(text)
+        cmd = ["gcc", "-O2", "-I", dd, "-I", INCLUDE, os.path.join(REFERENCE_ROOT, src), "-L", d, "-l" + name,
+               "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", driver_path(name)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("driver build failed for %s:\n%s\n%s" % (name, " ".join(cmd), r.stderr[-4000:]))
+    return lib
 
 
 # batched ensembles: name -> (program, config, precision, lifted literals, tile x, tile y, steps per launch)

@@ -69,8 +69,9 @@ def _eval_node(node, fields, temps, base, params=None, shape=None, offsets=None)
     return vals[-1]
 
 
-def step(mm_graph, sleeve, fields, params=None):
-    """One application of a step graph to a dict of periodic numpy fields."""
+def step(mm_graph, sleeve, fields, params=None, offsets=None):
+    """One application of a step graph to a dict of periodic numpy fields;
+    ``offsets``: the Navi offsets the kernel is called with (index reads)."""
     rng = calc_range(mm_graph)
     shape = next(iter(fields.values())).shape
     temps, out = {}, {}
@@ -78,7 +79,8 @@ def step(mm_graph, sleeve, fields, params=None):
         node = mm_graph[k]
         is_store = node.micro[-1].inst[0] == "Store"
         base = [rng[k].offset - 2 * sleeve] * len(shape) if is_store else [0] * len(shape)
-        v = np.broadcast_to(np.asarray(_eval_node(node, fields, temps, base, params, shape), dtype=np.float64), shape)
+        v = np.broadcast_to(np.asarray(_eval_node(node, fields, temps, base, params, shape, offsets),
+                                       dtype=np.float64), shape)
         if is_store:
             out[node.micro[-1].inst[1]] = np.array(v)
         else:

@@ -89,3 +89,32 @@ def test_sample_library_matches_fixture_bit_exact(name):
                                  % (name, f, float(np.max(np.abs(got - want[f])))))
     if name == "hllc1d_400":
         assert exact_interior_l1(lib.field("rho"), meta["steps"]) < 0.01
+
+
+@pytest.mark.gpu
+def test_reference_driver_links_unchanged(tmp_path):
+    """sample/1d_hllc/main.c of the reference, compiled unchanged against the
+    generated header and the HIP library (build.SAMPLE_DRIVERS): it runs
+    Formura_Init, writes data/0_0.dat, advances to time_step 100 with
+    Formura_Forward and writes data/100_0.dat.  Loaded with the drift-aware
+    loader, step 0 is the program's shock tube and step 100 is the fixture's
+    state, value for value (the files carry %.17e)."""
+    import subprocess
+    from formura_amd import output
+    exe = B.driver_path("hllc1d_400")
+    if not os.path.exists(exe):
+        pytest.skip("the reference's driver is built only where the formura checkout is present")
+    print("reference driver sample/1d_hllc/main.c on the HIP library", flush=True)
+    (tmp_path / "data").mkdir()
+    r = subprocess.run([exe], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d0 = output.load_data(str(tmp_path / "data"), 0, [400])[0, 0]
+    x = d0[:, 0]
+    np.testing.assert_array_equal(x, 0.01 * np.arange(400))
+    np.testing.assert_array_equal(d0[:, 3], np.where(np.arange(400) < 200, 1.0, 0.125))
+    meta, want = fixture("hllc1d_400")
+    d1 = output.load_data(str(tmp_path / "data"), 100, [400])[0, 0]
+    np.testing.assert_array_equal(d1[:, 0], 0.01 * np.arange(400))
+    rho = np.empty(400)
+    rho[(np.arange(400) + meta["offset"][0]) % 400] = want["rho"]
+    np.testing.assert_array_equal(d1[:, 3], rho)
