@@ -148,7 +148,7 @@ class ReverseSweep:
 
 class AdjointEmitter:
     def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256,
-                 tile=(16, 32, 3), adj_tile=None, levels=4):
+                 tile=(16, 32, 4), adj_tile=None, levels=4, row_block=1):
         self.mm, self.om, self.name = mm, mm.om, name
         self.real = real
         self.tile = tuple(tile) if tile else None
@@ -156,6 +156,8 @@ class AdjointEmitter:
         self.adj_tile = tuple(adj_tile) if adj_tile else (self.tile[:2] if self.tile else None)
         # most steps one fused adjoint launch reverses (2-D tiled kernels)
         self.levels = max(1, int(levels))
+        # points per work-item along the first axis in the adjoint levels kernel
+        self.row_block = int(row_block)
         self.dim = self.om.dimension
         self.N = self.om.config.grid_per_node
         self.params = list(params)
@@ -380,7 +382,8 @@ class AdjointEmitter:
             levels = 1
         desc["levels"] = levels
         return ADJ_TEMPLATE % dict(
-            name=self.name, R=R, nf=nf, np=np_, block=self.block, sleeve=s,
+            name=self.name, R=R, nf=nf, np=np_, has_params="true" if self.params else "false",
+            block=self.block, sleeve=s,
             consts="\n".join("constexpr int N%d = %d;" % (k + 1, n) for k, n in enumerate(self.N)),
             ntot="*".join("(long long)%d" % n for n in self.N), flds=flds, lams=lams,
             desc=json.dumps(desc, sort_keys=True), fwd="\n".join("  " + l for l in fwd), adj="\n".join("  " + l for l in body),
@@ -451,22 +454,22 @@ class AdjointEmitter:
                    "#pragma clang fp reassociate(on)\n#pragma clang fp contract(fast)\n%s\n}\n"
                    % (xs, ls, aouts, "\n".join("  " + l for l in pb + fin)))
         xl = "\n".join("  __shared__ real_t X%d[XH * XW];" % i for i in range(nf))
-        ll = "\n".join("  __shared__ real_t LA%d[LH * LW];\n  __shared__ real_t LB%d[LH * LW];" % (i, i) for i in range(nf))
+        ll = "\n".join("  __shared__ real_t L%d[LH * LW];" % i for i in range(nf))
         src.append(TILE_ADJ % dict(
             TX=ax, TY=ay, xl=xl, ll=ll, uses_state="true" if uses_state else "false",
             lload=tile_load("LH * LW", ["const int r = idx / LW, c = idx - r * LW;",
                                         "const long long g = (long long)wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);"],
-                            [("LA%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ll"),
+                            [("L%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ll"),
             xdecl=" ".join("real_t nx%d[NX_IT];" % i for i in range(nf)),
             xfetch="\n".join("          nx%d[it] = FLD%d(xs)[g];" % (i, i) for i in range(nf)),
-            xput="\n".join("        X%d[(idx / XWm) * XW + (idx - (idx / XWm) * XWm)] = nx%d[it];" % (i, i)
-                           for i in range(nf)),
-            sd="\n".join("    const real_t* S%d = (mm & 1) ? LB%d : LA%d; real_t* D%d = (mm & 1) ? LA%d : LB%d;"
-                         % (i, i, i, i, i, i) for i in range(nf)),
-            xargs=", ".join("X%d" % i for i in range(nf)), sargs=", ".join("S%d" % i for i in range(nf)),
+            xput="\n".join("        X%d[idx] = nx%d[it];" % (i, i) for i in range(nf)),
+            xargs=", ".join("X%d" % i for i in range(nf)), largs=", ".join("L%d" % i for i in range(nf)),
+            rdecl=" ".join("real_t res%d[NL_IT * RB];" % i for i in range(nf)),
+            rset=" ".join("res%d[it * RB + q] = a%d;" % (i, i) for i in range(nf)),
+            rb=self.row_block,
             adecl=" ".join("real_t a%d;" % i for i in range(nf)) + " " + " ".join("double q%d;" % p for p in range(np_)),
             aargs=", ".join(["a%d" % i for i in range(nf)] + ["q%d" % p for p in range(np_)]),
-            st_lds=" ".join("D%d[lo] = a%d;" % (i, i) for i in range(nf)),
+            st_lds=" ".join("L%d[lo] = res%d[it * RB + q];" % (i, i) for i in range(nf)),
             st_glb=" ".join("LAM%d(lam_prev)[g] = a%d;" % (i, i) for i in range(nf)),
             pdecl=" ".join("double pgs%d = 0;" % p for p in range(np_)),
             pacc=" ".join("pgs%d += q%d;" % (p, p) for p in range(np_)),
@@ -495,6 +498,11 @@ __global__ void __launch_bounds__(256) k_forward_tile(const char* __restrict__ s
 %(lds)s
   const int tile = xcd_tile(blockIdx.x, NT);
   const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+  // run-time parameters in registers for the whole launch (a scalar load per
+  // point would also drain the LDS reads in flight: both count in lgkmcnt)
+  double Pl[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) Pl[p] = HAS_PARAMS ? P[p] : 0.0;
 %(load_tile)s
   __syncthreads();
 #pragma unroll
@@ -507,7 +515,7 @@ __global__ void __launch_bounds__(256) k_forward_tile(const char* __restrict__ s
       const int r = lo + rr, c = lo + (idx - rr * W);
       const int o = r * RY + c;
       %(decl)s
-      pt_step<RY>(%(srcs)s, o, P, %(refs)s);
+      pt_step<RY>(%(srcs)s, o, Pl, %(refs)s);
       if (j < K) {
         %(st_lds)s
         if (ALL && r >= H && c >= H) {
@@ -540,62 +548,109 @@ __global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__
 %(ll)s
   const int tile = xcd_tile(blockIdx.x, NT);
   const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+  // run-time parameters in registers for the whole launch (a scalar load per
+  // point would also drain the LDS reads in flight: both count in lgkmcnt)
+  double Pl[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) Pl[p] = HAS_PARAMS ? P[p] : 0.0;
   // the state tile of the next level is fetched into registers (nx*) while the
-  // current level computes; it enters LDS once every read of the current one ended
+  // current level computes; it enters LDS once every read of the current one ended.
+  // Element it of this thread is (r, c) = divmod(idx, XW) of the largest region;
+  // its periodic global offset xg and the lowest level that reads it (level m
+  // reads rows < TX + 2(m+2)s and columns < TY + 2(m+2)s) are computed once.
   constexpr int NX_IT = (XH * XW + 255) / 256;
+  static_assert((long long)N1 * N2 < (1LL << 31), "element offsets of a field are ints");
   %(xdecl)s
+  int xg[NX_IT], xm[NX_IT];
+#pragma unroll
+  for (int it = 0; it < NX_IT; ++it) {
+    const int idx = threadIdx.x + it * 256;
+    const int r = idx / XW, c = idx - r * XW;
+    xg[it] = wrapn(x0 - 2 * SLEEVE + r, N1) * N2 + wrapn(y0 - 2 * SLEEVE + c, N2);
+    const int er = r - TX, ec = c - TY, e = er > ec ? er : ec;  // beyond the tile by e cells
+    xm[it] = idx < XH * XW ? (e < 4 * SLEEVE ? 0 : (e - 4 * SLEEVE) / (2 * SLEEVE) + 1) : K;
+  }
   auto xfetch = [&](int m) {
     const char* __restrict__ xs = m == 0 ? st0 : lv + (size_t)(m - 1) * STATE_BYTES;
-    const int XHm = TX + 2 * (m + 2) * SLEEVE, XWm = TY + 2 * (m + 2) * SLEEVE;
+#pragma unroll
+    for (int it = 0; it < NX_IT; ++it) {
+      if (xm[it] <= m) {
+        const int g = xg[it];
+%(xfetch)s
+      }
+    }
+  };
+  auto xput = [&](int m) {
 #pragma unroll
     for (int it = 0; it < NX_IT; ++it) {
       const int idx = threadIdx.x + it * 256;
-      if (idx < XHm * XWm) {
-        const int r = idx / XWm, c = idx - r * XWm;
-        const long long g = (long long)wrapn(x0 - 2 * SLEEVE + r, N1) * N2 + wrapn(y0 - 2 * SLEEVE + c, N2);
-%(xfetch)s
+      if (xm[it] <= m) {
+%(xput)s
       }
     }
   };
   if (USES_STATE) xfetch(K - 1);
 %(lload)s
+  if (USES_STATE) {
+    xput(K - 1);
+    if (K > 1) xfetch(K - 2);
+  }
+  __syncthreads();
   %(pdecl)s
+  // lam is updated in place: a level's values are computed into registers from
+  // lam_{m+1} in LDS, and written over it after every thread has read it
+  // each work-item evaluates RB vertically adjacent points of a column, so the
+  // LDS reads that their stencils share are issued once
+  constexpr int RB = %(rb)d;
+  static_assert(TX %% RB == 0 && (2 * SLEEVE) %% RB == 0, "row blocks tile every level");
+  constexpr int NL_IT = ((LH / RB) * LW + 255) / 256;
 #pragma unroll
   for (int mm = 0; mm < K; ++mm) {
     const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} -> lam_m at state X_m
-    if (USES_STATE) {
-      const int XHm = TX + 2 * (m + 2) * SLEEVE, XWm = TY + 2 * (m + 2) * SLEEVE;
-      __syncthreads();  // every read of the previous level's X is done
+    const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
+    %(rdecl)s
 #pragma unroll
-      for (int it = 0; it < NX_IT; ++it) {
-        const int idx = threadIdx.x + it * 256;
-        if (idx < XHm * XWm) {
-%(xput)s
+    for (int it = 0; it < NL_IT; ++it) {
+      // whole rows of the adjoint buffer, columns beyond the level masked: a
+      // wavefront's LDS accesses are contiguous (no bank conflicts at row wraps)
+      const int idx = threadIdx.x + it * 256;
+      const int rq = idx / LW, c = idx - rq * LW;
+      if (idx < (HH / RB) * LW && c < W) {
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+          const int r = rq * RB + q;
+          const bool own = r < TX && c < TY;
+          %(adecl)s
+          adj_point<XW, LW>(%(xargs)s, %(largs)s, (r + 2 * SLEEVE) * XW + (c + 2 * SLEEVE), r * LW + c, own, Pl, %(aargs)s);
+          if (m > 0) {
+            %(rset)s
+          } else {
+            const long long g = (long long)(x0 + r) * N2 + (y0 + c);
+            %(st_glb)s
+          }
+          if (own) { %(pacc)s }
         }
       }
     }
-    __syncthreads();
-    if (USES_STATE && m > 0) xfetch(m - 1);
-%(sd)s
-    const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
-    for (int idx = threadIdx.x; idx < HH * LW; idx += 256) {
-      // whole rows of the adjoint buffers, columns beyond the level masked: a
-      // wavefront's LDS accesses are contiguous (no bank conflicts at row wraps);
-      // measured faster here, while the forward kernels, bound by VALU, lose more
-      // to the masked lanes than they gain
-      const int r = idx / LW, c = idx - r * LW;
-      if (c >= W) continue;
-      const bool own = r < TX && c < TY;
-      %(adecl)s
-      adj_point<XW, LW>(%(xargs)s, %(sargs)s, (r + 2 * SLEEVE) * XW + (c + 2 * SLEEVE), r * LW + c, own, P, %(aargs)s);
-      if (m > 0) {
-        const int lo = r * LW + c;
-        %(st_lds)s
-      } else {
-        const long long g = (long long)(x0 + r) * N2 + (y0 + c);
-        %(st_glb)s
+    if (m > 0) {
+      __syncthreads();  // every read of lam_{m+1} and X_m is done
+#pragma unroll
+      for (int it = 0; it < NL_IT; ++it) {
+        const int idx = threadIdx.x + it * 256;
+        const int rq = idx / LW, c = idx - rq * LW;
+        if (idx < (HH / RB) * LW && c < W) {
+#pragma unroll
+          for (int q = 0; q < RB; ++q) {
+            const int lo = (rq * RB + q) * LW + c;
+            %(st_lds)s
+          }
+        }
       }
-      if (own) { %(pacc)s }
+      if (USES_STATE) {
+        xput(m - 1);
+        if (m > 1) xfetch(m - 2);
+      }
+      __syncthreads();
     }
   }
   { const double pv[NP] = {%(pgs)s}; block_reduce_params<256>(pgrad + (long long)blockIdx.x * NP, pv); }
@@ -649,6 +704,7 @@ typedef %(R)s real_t;
 constexpr long long NTOT = %(ntot)s;
 constexpr int NF = %(nf)d;
 constexpr int NP = %(np)d;
+constexpr bool HAS_PARAMS = %(has_params)s;
 constexpr int BLOCK = %(block)d;
 constexpr int SLEEVE = %(sleeve)d;
 constexpr int KA = %(levels)d;  // most adjoint steps per Formura_AD_Adjoint_Steps call

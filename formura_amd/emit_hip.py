@@ -440,8 +440,12 @@ class Tile2DSchedule:
 
     name = "tile2d"
 
-    def __init__(self, tx, ty, k):
+    def __init__(self, tx, ty, k, inplace=False):
         self.tx, self.ty, self.k = tx, ty, k
+        # in place: one LDS buffer per field; a level's values go to registers
+        # and overwrite the previous level after a barrier (half the LDS, so
+        # more workgroups per CU, for one more barrier per level)
+        self.inplace = bool(inplace)
 
     def kernels(self, em, plan):
         out = []
@@ -474,7 +478,8 @@ class Tile2DSchedule:
                    % (params, outs, "\n".join("  " + l for l in body_lines)))
         wrap1 = em.wrap_fn(0, 2 * self.k * s + 1)
         wrap2 = em.wrap_fn(1, 2 * self.k * s + 1)
-        lds = "\n".join("  __shared__ %s B%d[2][RX * RY];" % (ct, i) for i, (n, ct) in enumerate(fields))
+        nb = 1 if self.inplace else 2
+        lds = "\n".join("  __shared__ %s B%d[%d][RX * RY];" % (ct, i, nb) for i, (n, ct) in enumerate(fields))
         load_tile = tile_load("RX * RY", ["const int r = idx / RY, c = idx - r * RY;",
                                           "const long long g = (long long)%s(x0 - H + r) * N2 + %s(y0 - H + c);"
                                           % (wrap1, wrap2)],
@@ -485,6 +490,15 @@ class Tile2DSchedule:
         refs_o = ", ".join("o%d" % i for i in range(len(fields)))
         st_lds = " ".join("B%d[dst][o] = o%d;" % (i, i) for i in range(len(fields)))
         st_glb = " ".join("FLD%d(st_out)[g] = o%d;" % (i, i) for i in range(len(fields)))
+        if self.inplace:
+            out.append(TILE2D_INPLACE % dict(
+                TX=TX, TY=TY, lds=lds, load_tile=load_tile, decl_o=decl_o, refs_o=refs_o,
+                srcs=", ".join("B%d[0]" % i for i in range(len(fields))),
+                rdecl=" ".join("%s res%d[NI];" % (ct, i) for i, (n, ct) in enumerate(fields)),
+                rset=" ".join("res%d[it] = o%d;" % (i, i) for i in range(len(fields))),
+                rput=" ".join("B%d[0][o] = res%d[it];" % (i, i) for i in range(len(fields))),
+                st_glb=st_glb))
+            return out
         out.append("""
 template <int K>
 __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
@@ -537,6 +551,60 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
                 "    const int k = steps >= %d ? %d : (int)steps;\n"
                 "    run_step_tile(k, g_rt.cur, g_rt.nxt, g_rt.stream); g_rt.swap();\n"
                 "    g_rt.launches += 1; steps -= k;\n  }\n}" % (self.k, self.k))
+
+
+TILE2D_INPLACE = """
+template <int K>
+__global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
+  constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
+  constexpr int NTX = N1 / TX, NTY = N2 / TY, NT = NTX * NTY;
+%(lds)s
+  // XCD-aware tile order: workgroup ids are dealt round-robin to the 8 XCDs, so
+  // give each XCD a contiguous band of tiles (shared halos stay in its L2).
+  const int b = blockIdx.x;
+  const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
+  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+%(load_tile)s
+  __syncthreads();
+  // points of the largest level per thread; level j covers [2js, RX) x [2js, RY)
+  constexpr int NI = ((RX - 2 * SLEEVE) * (RY - 2 * SLEEVE) + 255) / 256;
+#pragma unroll
+  for (int j = 1; j <= K; ++j) {
+    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
+    %(rdecl)s
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      const int idx = threadIdx.x + it * 256;
+      if (idx < HH * W) {
+        const int rr = idx / W;
+        const int r = lo + rr, c = lo + (idx - rr * W);
+        const int o = r * RY + c;
+        %(decl_o)s
+        pt_step<RY>(%(srcs)s, o, %(refs_o)s);
+        if (j < K) {
+          %(rset)s
+        } else {
+          const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+          %(st_glb)s
+        }
+      }
+    }
+    if (j < K) {
+      __syncthreads();  // every read of level j-1 is done
+#pragma unroll
+      for (int it = 0; it < NI; ++it) {
+        const int idx = threadIdx.x + it * 256;
+        if (idx < HH * W) {
+          const int rr = idx / W;
+          const int o = (lo + rr) * RY + lo + (idx - rr * W);
+          %(rput)s
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+"""
 
 
 class Stream3DSchedule:
@@ -716,7 +784,7 @@ def schedule_from_spec(em, spec: dict):
         args = (int(spec["tx"]), int(spec["ty"]), int(spec["k"]))
         if not _tile2d_ok(em, *args):
             raise EmitError("tile2d %r does not apply to %s" % (spec, em.name))
-        return Tile2DSchedule(*args)
+        return Tile2DSchedule(*args, inplace=bool(spec.get("inplace", 0)))
     if kind == "stream3d":
         args = (int(spec["ty"]), int(spec["tz"]), int(spec["cx"]), int(spec["k"]))
         if not _stream3d_ok(em, *args):
@@ -747,6 +815,8 @@ def candidate_specs(em) -> List[dict]:
             for k in (1, 2, 3, 4, 6, 8):
                 if _tile2d_ok(em, tx, ty, k):
                     out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k})
+                    if k > 1:
+                        out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k, "inplace": 1})
     if em.dim == 3:
         for ty, tz in ((16, 64), (8, 64), (32, 32), (16, 32), (8, 32), (4, 64), (4, 32), (8, 16), (4, 16)):
             for cx in (8, 16, 32, 64):

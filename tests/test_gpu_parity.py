@@ -97,6 +97,7 @@ def test_resident_multi_step_launches(name, steps):
     ("heat3d_64_tb_stream", "heat3d_64_tb", 7, True),  # 4 steps per Forward, 3 per launch
     ("gs_64_flat", "gs_64", 33, False),
     ("gs_64_tile_k8", "gs_64", 61, True),             # 8 steps per launch, halo 16 of a 64 grid
+    ("gs_64_tile_k5_inplace", "gs_64", 47, True),     # one LDS buffer per field, levels overwritten in place
 ])
 def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     seed = (lambda lib: inputs.heat_field(13, lib.grid)) if name.startswith("heat") else _gs_seed
