@@ -341,7 +341,9 @@ def run_samples(steps: int = 50):
     from formura_amd import build as B, inputs
     from formura_amd.runtime import open_program
     out = {"steps": steps}
-    for name, gamma in (("mhd3d_128", 5.0 / 3.0), ("hllc3d_128", 7.0 / 5.0)):
+    finals = {}
+    for name, gamma in (("mhd3d_128", 5.0 / 3.0), ("mhd3d_128_mixed", 5.0 / 3.0),
+                        ("hllc3d_128", 7.0 / 5.0), ("hllc3d_128_mixed", 7.0 / 5.0)):
         if not os.path.exists(B.lib_path(name)):
             out[name] = {"skipped": "built only where the formura checkout is present"}
             continue
@@ -358,9 +360,18 @@ def run_samples(steps: int = 50):
         cons = [f for f in t0 if f != "psi"]
         out[name] = {"glups": cells * steps / (ms * 1e-3) / 1e9, "ms": ms, "schedule": lib.layout.get("schedule"),
                      "fields": len(lib.fields),
-                     "max_rel_total_drift": max(abs(t1[f] - t0[f]) for f in cons) / max(abs(t0[f]) for f in cons)}
+                     "max_rel_total_drift": max(abs(t1[f] - t0[f]) for f in cons) / max(abs(t0[f]) for f in cons),
+                     "positivity_violations": lib.positivity_violations()}
+        lib.download()
+        finals[name] = {f: lib.field(f).copy() for f in lib.fields}
         lib.set_resident(False)
         lib.finalize()
+    for base in ("mhd3d_128", "hllc3d_128"):
+        a, b = finals.get(base), finals.get(base + "_mixed")
+        if a and b:
+            out[base + "_mixed"]["rel_maxnorm_err_vs_fp64"] = {
+                f: float(np.max(np.abs(a[f] - b[f])) / np.max(np.abs(a[f]))) for f in a}
+            out[base + "_mixed"]["speedup_vs_fp64"] = out[base + "_mixed"]["glups"] / out[base]["glups"]
     return out
 
 

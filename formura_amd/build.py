@@ -81,6 +81,11 @@ SAMPLES: Dict[str, tuple] = {
     "hllc3d_8": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_8.yaml", "auto"),
     "mhd3d_128": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_128.yaml", "auto"),
     "hllc3d_128": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_128.yaml", "auto"),
+    # mixed form: FP32 fluxes, FP64 wave speeds, reconstructions, positivity checks and update
+    "mhd3d_8_mixed": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_8.yaml", {"kind": "flat", "mixed": 1}),
+    "hllc3d_8_mixed": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_8.yaml", {"kind": "flat", "mixed": 1}),
+    "mhd3d_128_mixed": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_128.yaml", {"kind": "flat", "mixed": 1}),
+    "hllc3d_128_mixed": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_128.yaml", {"kind": "flat", "mixed": 1}),
 }
 
 

@@ -377,6 +377,75 @@ def mixed_precision(node: MMNode) -> Dict[int, str]:
     return {j: ("double" if fp64[j] else "float") for j in range(len(micro))}
 
 
+def mixed_precision_graph(mm: Dict[int, MMNode]):
+    """The mixed form over a step graph whose manifest temporaries stay
+    materialized (one buffer per temporary, as the reference's
+    ``formura_mnX``).  Returns (precision of every micro value per node,
+    storage type per temporary).
+
+    The rules of ``mixed_precision`` hold across nodes: a value that decides a
+    branch or feeds a square root is FP64 with all its operands, and when such
+    an operand is a temporary read (LoadCursor) the temporary's own result is
+    FP64 with all its operands, in its node, and so on; state loads are FP64;
+    the update chain of a stored value (+ - * / and unary minus) is FP64 down
+    to the reads it reaches, which are converted on load.  Every other value,
+    and every temporary whose result is not FP64, is FP32: an approximate
+    Riemann solver keeps its wave speeds and the densities and pressures they
+    come from in FP64 and computes and stores its fluxes in FP32.  A face flux
+    stays one stored value that both neighbouring cells subtract, so totals are
+    still conserved to FP64 rounding."""
+    fp64 = {k: [False] * len(n.micro) for k, n in mm.items()}
+    closed = {k: [False] * len(n.micro) for k, n in mm.items()}
+    work = []
+
+    def run():
+        while work:
+            k, j = work.pop()
+            if j is None or closed[k][j]:
+                continue
+            closed[k][j] = True
+            fp64[k][j] = True
+            m = mm[k].micro[j]
+            if m.inst[0] == "LoadCursor" and m.inst[2] in mm:
+                t = m.inst[2]
+                work.append((t, len(mm[t].micro) - 1))
+            work.extend((k, o) for o in _operands(m) if o is not None)
+
+    for k, node in mm.items():
+        micro = node.micro
+        const = [False] * len(micro)
+        for j, m in enumerate(micro):
+            kind = m.inst[0]
+            if kind == "Binop" and m.inst[1] in COMPARISONS:
+                work.append((k, j))
+            elif kind == "Triop":
+                work.append((k, m.inst[2]))
+            elif kind == "Uniop" and m.inst[1] == "external-call/sqrt":
+                work.append((k, j))
+            elif kind == "LoadCursorStatic":
+                fp64[k][j] = True
+            ops = _operands(m)
+            const[j] = kind == "Imm" or (kind in ("Uniop", "Binop", "Triop") and all(
+                o is not None and const[o] for o in ops))
+            if const[j]:
+                fp64[k][j] = True
+        st = micro[-1]
+        if st.inst[0] == "Store":
+            stack = [st.inst[2]]
+            while stack:
+                j = stack.pop()
+                m = micro[j]
+                if (m.inst[0] == "Binop" and m.inst[1] in ("+", "-", "*", "/")) or (m.inst[0] == "Uniop" and m.inst[1] == "-"):
+                    fp64[k][j] = True
+                    stack += _operands(m)
+                elif m.inst[0] == "LoadCursor":
+                    fp64[k][j] = True
+    run()
+    prec = {k: {j: ("double" if v else "float") for j, v in enumerate(f)} for k, f in fp64.items()}
+    storage = {k: ("double" if fp64[k][-1] else "float") for k, n in mm.items() if n.micro[-1].inst[0] != "Store"}
+    return prec, storage
+
+
 def _operands(m):
     k = m.inst[0]
     if k == "Uniop":
@@ -442,8 +511,10 @@ def emit_header(om, with_ext: bool = True) -> str:
 class GraphPlan:
     """What one MM graph (step, filter or first_step) needs on the GPU."""
 
-    def __init__(self, mm: Dict[int, MMNode], sleeve: int, dim: int, tag: str):
+    def __init__(self, mm: Dict[int, MMNode], sleeve: int, dim: int, tag: str, storage: Dict[int, str] = None):
         self.mm = mm
+        # storage type of each manifest temporary (mixed form), else its C type
+        self.storage = storage
         self.sleeve = sleeve
         self.tag = tag
         self.rng = calc_range(mm)
@@ -468,6 +539,8 @@ class GraphPlan:
                         self.max_reach = max(self.max_reach, abs(c + base))
 
     def temp_ctype(self, k):
+        if self.storage and k in self.storage and c_elem_type(self.mm[k].typ) == "double":
+            return self.storage[k]
         return c_elem_type(self.mm[k].typ)
 
 

@@ -24,7 +24,7 @@ import json
 from typing import Dict, List
 
 from . import syntax as S
-from .emit import (tile_load, mixed_precision, Body, EmitError, GraphPlan, c_elem_type, emit_header,
+from .emit import (tile_load, mixed_precision, mixed_precision_graph, Body, EmitError, GraphPlan, c_elem_type, emit_header,
                    lib_description, navi_fields, show_double, state_fields, CTYPE)
 from .manifest import MMNode, calc_range
 
@@ -61,6 +61,11 @@ class LibEmitter:
         # node before any inlining
         self.mixed = bool(isinstance(spec, dict) and spec.get("mixed"))
         self._prec = {k: mixed_precision(n) for k, n in mm.step.items()} if self.mixed else None
+        storage = None
+        if self.mixed and not self.inlined:
+            # temporaries stay materialized: precision decided over the whole graph
+            self._prec, storage = mixed_precision_graph(mm.step)
+        step_mm = mm.step
         if self.inlined:
             from .manifest import inline_temporaries
             orig_rng = calc_range(mm.step)
@@ -69,7 +74,7 @@ class LibEmitter:
             for k in step_mm:
                 if (orig_rng[k].lower, orig_rng[k].upper) != (new_rng[k].lower, new_rng[k].upper):
                     raise EmitError("inlining changed the stencil range of %s" % step_mm[k].name)
-        self.step = GraphPlan(step_mm, self.cfg.sleeve, self.dim, "step")
+        self.step = GraphPlan(step_mm, self.cfg.sleeve, self.dim, "step", storage=storage)
         self.filter = GraphPlan(mm.filter, self.cfg.filter_sleeve, self.dim, "filter") if mm.filter else None
         self.first = GraphPlan(mm.first_step, self.cfg.sleeve0, self.dim, "first_step") if mm.first_step else None
         if self.cfg.blocking is not None:
@@ -135,7 +140,7 @@ class LibEmitter:
         for k in plan.temps:
             node = plan.mm[k]
             load, tmp_load = loaders([0] * dim)
-            body = Body(node, load, tmp_load, lambda ax: "0", lambda n, v: "", "a")
+            body = Body(node, load, tmp_load, lambda ax: "0", lambda n, v: "", "a", prec=self.prec_of(node))
             body.lines.append("t_%d[t] = %s;" % (k, body.result))
             deps = [d for d in plan.temps if d < k]
             params = "".join(", const %s* __restrict__ t_%d" % (plan.temp_ctype(d), d) for d in deps)

@@ -118,3 +118,36 @@ def test_reference_driver_links_unchanged(tmp_path):
     rho = np.empty(400)
     rho[(np.arange(400) + meta["offset"][0]) % 400] = want["rho"]
     np.testing.assert_array_equal(d1[:, 3], rho)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["mhd3d_8", "hllc3d_8"])
+def test_mixed_sample_against_fp64(name):
+    """The mixed form of the 3-D samples (FP32 fluxes; FP64 wave speeds,
+    reconstructions, square-root arguments and the update chain): after the
+    fixture's steps every field is within 1e-5 (relative max-norm) of the FP64
+    run, every total but psi is conserved to 1e-13 of its scale, and no FP64
+    square-root argument was found non-positive."""
+    path = B.lib_path(name + "_mixed")
+    if not os.path.exists(path):
+        pytest.skip("%s_mixed is built only where the formura checkout is present" % name)
+    meta, want = fixture(name)
+    print("mixed sample %s: %d steps" % (name, meta["steps"]), flush=True)
+    lib = load_lib(path)
+    lib.init()
+    st = inputs.conserved_state(meta["seed"], tuple(meta["grid"]), lib.fields, meta["gamma"])
+    lib.set_state(st)
+    lib.set_resident(True)
+    lib.upload()
+    t0 = lib.totals()
+    lib.forward_steps(meta["steps"])
+    t1 = lib.totals()
+    lib.download()
+    lib.set_resident(False)
+    assert lib.positivity_violations() == 0
+    for f in lib.fields:
+        err = float(np.max(np.abs(lib.field(f) - want[f])) / np.max(np.abs(want[f])))
+        assert err < 1e-5, (f, err)
+    cons = [f for f in t0 if f != "psi"]
+    scale = max(abs(t0[f]) for f in cons)
+    assert max(abs(t1[f] - t0[f]) for f in cons) <= 1e-13 * scale
