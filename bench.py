@@ -375,6 +375,22 @@ def run_samples(steps: int = 50):
     return out
 
 
+def run_adjoint_probe(nsteps: int = 200, snapshots: int = 8):
+    """A short FP64 Gray-Scott 1024^2 forward + backward (for profiling the adjoint
+    kernels without the finite-difference checks of the adjoint leg)."""
+    import numpy as np
+    import torch
+    from formura_amd import adjoint as A, batch as B
+    st = B.grayscott_init(1024, 11)
+    s = A.Stepper("gs_1024_adj")
+    x = torch.tensor(np.stack([st["u"], st["v"]]), device="cuda").requires_grad_(True)
+    p = torch.tensor([0.04, 0.06, 2e-5, 1e-5], dtype=torch.float64, requires_grad=True)
+    y = A.run(s, x, p, nsteps, snapshots)
+    (0.5 * (y[1] ** 2).sum()).backward()
+    torch.cuda.synchronize()
+    return {"steps": nsteps, "snapshots": snapshots, "dL_dF": float(p.grad[0]), "stats": dict(s.last_stats)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,7 +405,8 @@ def main():
                     help="skip the heat 512^3, Euler 256^3 and adjoint legs")
     ap.add_argument("--legs", default="plain,ensemble,heat,euler,adjoint,samples,cpu",
                     help="comma-separated legs to run (for profiling one kernel family): plain (Gray-Scott "
-                         "2048^2, the headline value), ensemble, heat, euler, adjoint, samples, cpu")
+                         "2048^2, the headline value), ensemble, heat, euler, adjoint, samples, cpu; "
+                         "adjoint_probe (a short adjoint run for profiling, not in the default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB, 20 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -425,7 +442,8 @@ def main():
     if "plain" not in legs:
         line = {"legs": sorted(legs)}
         for key, leg, fn in (("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
-                             ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples)):
+                             ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples),
+                             ("adjoint_probe", "adjoint_probe", run_adjoint_probe)):
             if leg in legs:
                 line[key] = fn()
         if "ensemble" in legs:
