@@ -56,6 +56,8 @@ PROGRAMS: Dict[str, tuple] = {
                                  {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 1, "inline": 1}),
     "gs_64_tile_k8": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 8}),
     "gs_64_tile_k5_inplace": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 5, "inplace": 1}),
+    "gs_64_wave_k3": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "wave2d", "rows": 16, "k": 3}),
+    "gs_2048_wave": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "wave2d", "rows": 32, "k": 3}),
     # mixed precision: FP32 flux arithmetic, FP64 wave speeds, positivity checks and update
     "euler3d_16_mixed": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),
     "euler3d_32_mixed": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),

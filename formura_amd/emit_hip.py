@@ -612,6 +612,193 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
 """
 
 
+class Wave2DSchedule:
+    """2-D, no temporaries: one wavefront streams a 64-column strip along x
+    and keeps every level in registers (no LDS, no barriers).
+
+    Lane l of a wave holds column c0 - H + l (H = 2*K*s), so the strip's
+    outputs are lanes H..63 and the lower lanes are its low-side halo.  The
+    wave walks the rows r0 - H .. r0 + R - 1 of its chunk.  With the drifting
+    frame an output (r, c) reads its input at rows r-2s..r and columns
+    c-2s..c, so all K levels of row r can be computed in the iteration that
+    loads input row r: level j reads level j-1's rows r-2s..r from a short
+    register window, and a column offset -d is the same window shifted by d
+    lanes (wave_shr:1 DPP moves, applied once when a row enters the window).
+    Loads and stores are whole rows of 64 consecutive values.  Every point is
+    evaluated with the same instruction list as the one-step kernel, so the
+    result is identical to K single steps.  Halo lanes and warm-up rows
+    compute values that are never stored."""
+
+    name = "wave2d"
+
+    def __init__(self, rows, k, unroll=1):
+        self.rows, self.k = rows, k
+        # rows per loop trip: the rows of one trip at the same level are
+        # independent, so unrolling gives the scheduler two chains to interleave
+        self.unroll = int(unroll)
+
+    def kernels(self, em, plan):
+        s = plan.sleeve
+        K, R = self.k, self.rows
+        H = 2 * K * s
+        fields = em.fields
+        nf = len(fields)
+        # which (field, row offset, column offset) each level reads
+        uses = {}
+        for k in plan.stores:
+            r = plan.rng[k]
+            base = r.offset - 2 * s
+            for m in plan.mm[k].micro:
+                if m.inst[0] == "LoadCursorStatic":
+                    fi = em.field_index[m.inst[2]]
+                    dr, dc = m.inst[1][0] + base, m.inst[1][1] + base
+                    if not (-2 * s <= dr <= 0 and -2 * s <= dc <= 0):
+                        raise EmitError("wave2d: load offset (%d,%d) outside [-2s,0]" % (dr, dc))
+                    uses.setdefault((fi, -dc), set()).add(-dr)
+        depth = {key: max(v) + 1 for key, v in uses.items()}
+        shifts = {fi: sorted(d for (f, d) in uses if f == fi) for fi in range(nf)}
+        out = []
+        shr = []
+        for ct in sorted(set(ct for _, ct in fields)):
+            if ct == "double":
+                shr.append("""__device__ __forceinline__ double fmr_shr1(double x) {
+  // lane l receives lane l-1 (lane 0 receives 0): one wave_shr:1 DPP move per 32-bit half
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}""")
+            elif ct == "float":
+                shr.append("""__device__ __forceinline__ float fmr_shr1(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
+}""")
+        out.append("\n".join(shr) + "\n")
+
+        def wname(lvl, fi, d, row):
+            return "w%d_%d_%d_%d" % (lvl, fi, d, row)
+
+        def level_body(lvl_in, prefix):
+            """Instructions computing level lvl_in+1 from the windows of level lvl_in."""
+            lines = []
+            for k in plan.stores:
+                node = plan.mm[k]
+                r = plan.rng[k]
+                base = r.offset - 2 * s
+
+                def load(field, c, base=base):
+                    fi = em.field_index[field]
+                    return wname(lvl_in, fi, -(c[1] + base), -(c[0] + base))
+
+                def store(fname, v):
+                    return "n%d = %s;" % (em.field_index[fname], v)
+
+                b = Body(node, load, lambda *a: None, lambda ax: None, store, "%s%d_" % (prefix, k),
+                         prec=em.prec_of(node))
+                lines.append("{")
+                lines += ["  " + l for l in b.lines]
+                lines.append("}")
+            return lines
+
+        def push(lvl, src_of):
+            """Rotate level lvl's windows and enter the new row (n<f>) with its shifts."""
+            lines = []
+            for fi in range(nf):
+                ct = fields[fi][1]
+                prev = src_of(fi)
+                for d in range(0, max(shifts[fi] or [0]) + 1):
+                    cur = "sh%d_%d" % (fi, d)
+                    lines.append("const %s %s = %s;" % (ct, cur, prev if d == 0 else "fmr_shr1(%s)" % prev))
+                    prev = cur
+                    if d not in shifts[fi]:
+                        continue
+                    dep = depth[(fi, d)]
+                    for row in range(dep - 1, 0, -1):
+                        lines.append("%s = %s;" % (wname(lvl, fi, d, row), wname(lvl, fi, d, row - 1)))
+                    lines.append("%s = %s;" % (wname(lvl, fi, d, 0), cur))
+            return lines
+
+        decl = []
+        for lvl in range(K):
+            for (fi, d), dep in sorted(depth.items()):
+                ct = fields[fi][1]
+                decl.append(" ".join("%s %s = 0;" % (ct, wname(lvl, fi, d, row)) for row in range(dep)))
+        body = []
+        # level 0: the input row (prefetched one iteration ahead)
+        body.append("{")
+        body += ["  " + l for l in push(0, lambda fi: "pf%d" % fi)]
+        body.append("}")
+        body.append("if (i + 1 < R) {")
+        body.append("  const long long g = (long long)fmr_wrap<N1>(r0 + i + 1) * N2 + col;")
+        body += ["  pf%d = FLD%d(st_in)[g];" % (fi, fi) for fi in range(nf)]
+        body.append("}")
+        # levels 1..K of the launch's template K (<= the schedule's K): level j
+        # enters its window unless it is the last, which is stored
+        for j in range(1, K + 1):
+            body.append("if constexpr (%d <= K) {  // level %d" % (j, j))
+            body.append("  " + " ".join("%s n%d;" % (ct, fi) for fi, (n, ct) in enumerate(fields)))
+            body += ["  " + l for l in level_body(j - 1, "l%d_" % j)]
+            if j < K:
+                body.append("  if constexpr (%d < K) {" % j)
+                body += ["    " + l for l in push(j, lambda fi: "n%d" % fi)]
+                body.append("  }")
+            body.append("  if constexpr (%d == K) {" % j)
+            body.append("    if (i >= 0 && lane >= H) {")
+            body.append("      const long long g = (long long)(r0 + i) * N2 + col;")
+            body += ["      FLD%d(st_out)[g] = n%d;" % (fi, fi) for fi in range(nf)]
+            body.append("    }")
+            body.append("  }")
+            body.append("}")
+        out.append("""
+template <int K>
+__global__ void __launch_bounds__(256) k_step_wave(const char* __restrict__ st_in, char* __restrict__ st_out) {
+  constexpr int H = 2 * K * SLEEVE, WOUT = 64 - H, R = %(R)d;
+  constexpr int NSTRIP = (N2 + WOUT - 1) / WOUT, NCHUNK = N1 / R;
+  const int lane = threadIdx.x & 63;
+  const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wv >= NSTRIP * NCHUNK) return;  // whole waves only
+  const int strip = wv %% NSTRIP, chunk = wv / NSTRIP;
+  // the last strip is moved back to end at column N2 (it recomputes columns of its
+  // neighbour, which both store with identical values)
+  const int c0 = strip * WOUT + WOUT <= N2 ? strip * WOUT : N2 - WOUT;
+  const int col = fmr_wrap<N2>(c0 - H + lane);
+  const int r0 = chunk * R;
+  %(decl)s
+  %(pfdecl)s
+  {
+    const long long g = (long long)fmr_wrap<N1>(r0 - H) * N2 + col;
+    %(pfload)s
+  }
+#pragma unroll %(unroll)d
+  for (int i = -H; i < R; ++i) {
+%(body)s
+  }
+}
+""" % dict(R=R, unroll=self.unroll, decl="\n  ".join(decl),
+           pfdecl=" ".join("%s pf%d;" % (ct, fi) for fi, (n, ct) in enumerate(fields)),
+           pfload=" ".join("pf%d = FLD%d(st_in)[g];" % (fi, fi) for fi in range(nf)),
+           body="\n".join("    " + l for l in body)))
+        return out
+
+    def nwaves(self, em, k):
+        wout = 64 - 2 * k * em.cfg.sleeve
+        return ((em.N[1] + wout - 1) // wout) * (em.N[0] // self.rows)
+
+    def launch(self, em, plan):
+        return ("static void run_step_wave(int k, const char* in, char* out, hipStream_t s) {\n"
+                "  switch (k) {\n%s  default: fmr_rt::fatal(\"bad step count\");\n  }\n"
+                "  FMR_CHECK(hipGetLastError());\n}\n"
+                % "".join("  case %d: hipLaunchKernelGGL(k_step_wave<%d>, dim3(%d), dim3(256), 0, s, in, out); break;\n"
+                          % (k, k, (self.nwaves(em, k) + 3) // 4) for k in range(1, self.k + 1)))
+
+    def advance(self, em):
+        return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
+                "  (void)n0;\n"
+                "  while (steps > 0) {\n"
+                "    const int k = steps >= %d ? %d : (int)steps;\n"
+                "    run_step_wave(k, g_rt.cur, g_rt.nxt, g_rt.stream); g_rt.swap();\n"
+                "    g_rt.launches += 1; steps -= k;\n  }\n}" % (self.k, self.k))
+
+
 class Stream3DSchedule:
     """3-D, no temporaries: stream along the slowest axis with K levels in LDS.
 
@@ -768,6 +955,16 @@ def _stream3d_ok(em, ty, tz, cx, k):
     return lds <= 64 * 1024
 
 
+def _wave2d_ok(em, rows, k):
+    plan = em.step
+    if em.dim != 2 or plan.temps or em.cfg.blocking is not None and False:
+        return False
+    if any(m.inst[0] == "LoadIndex" for kk in plan.stores for m in plan.mm[kk].micro):
+        return False
+    h = 2 * k * plan.sleeve
+    return h < 48 and em.N[1] >= 64 and em.N[0] % rows == 0 and em.N[0] > h and em.N[1] > h
+
+
 def _tile2d_ok(em, tx, ty, k):
     plan = em.step
     if em.dim != 2 or plan.temps:
@@ -790,6 +987,11 @@ def schedule_from_spec(em, spec: dict):
         if not _tile2d_ok(em, *args):
             raise EmitError("tile2d %r does not apply to %s" % (spec, em.name))
         return Tile2DSchedule(*args, inplace=bool(spec.get("inplace", 0)))
+    if kind == "wave2d":
+        args = (int(spec["rows"]), int(spec["k"]))
+        if not _wave2d_ok(em, *args):
+            raise EmitError("wave2d %r does not apply to %s" % (spec, em.name))
+        return Wave2DSchedule(*args, unroll=int(spec.get("unroll", 1)))
     if kind == "stream3d":
         args = (int(spec["ty"]), int(spec["tz"]), int(spec["cx"]), int(spec["k"]))
         if not _stream3d_ok(em, *args):
@@ -822,6 +1024,11 @@ def candidate_specs(em) -> List[dict]:
                     out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k})
                     if k > 1:
                         out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k, "inplace": 1})
+        for rows in (8, 16, 32, 64):
+            for k in (1, 2, 3, 4):
+                if _wave2d_ok(em, rows, k):
+                    out.append({"kind": "wave2d", "rows": rows, "k": k})
+                    out.append({"kind": "wave2d", "rows": rows, "k": k, "unroll": 2})
     if em.dim == 3:
         for ty, tz in ((16, 64), (8, 64), (32, 32), (16, 32), (8, 32), (4, 64), (4, 32), (8, 16), (4, 16)):
             for cx in (8, 16, 32, 64):

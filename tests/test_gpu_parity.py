@@ -98,6 +98,8 @@ def test_resident_multi_step_launches(name, steps):
     ("gs_64_flat", "gs_64", 33, False),
     ("gs_64_tile_k8", "gs_64", 61, True),             # 8 steps per launch, halo 16 of a 64 grid
     ("gs_64_tile_k5_inplace", "gs_64", 47, True),     # one LDS buffer per field, levels overwritten in place
+    ("gs_64_wave_k3", "gs_64", 47, True),             # register windows along x, DPP lane shifts, no LDS
+    ("gs_2048_wave", "gs_2048", 31, True),            # the same at full size (strips overlap at the last column)
 ])
 def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     seed = (lambda lib: inputs.heat_field(13, lib.grid)) if name.startswith("heat") else _gs_seed
