@@ -119,6 +119,28 @@ def run_ensemble(count: int, steps: int):
     out["f32k_rel_l2_vs_f64"] = {"max": float(errs.max()), "median": float(np.median(errs)),
                                  "instances_le_1e-5": int((errs <= 1e-5).sum())}
     out["f32k_max_rms_error_per_field"] = rms
+    # Control: the FP64 ensemble again, from initial states perturbed at the scale
+    # of FP32 rounding (u scaled by 1 + 2^-24 (2r - 1), r ~ U[0,1) seeded).  An
+    # instance whose FP64 run moves by more than 1e-5 under that perturbation
+    # amplifies rounding-size differences; FP32 arithmetic cannot follow it.
+    from formura_amd import inputs
+    ref = {i: ens["gs_batch_f64"].get_state(i) for i in range(count)}
+    e = ens["gs_batch_f64"]
+    for i, st in enumerate(inits):
+        r = inputs.uniform(977 + i, st["u"].shape)
+        e.set_state(i, {"u": st["u"] * (1.0 + 2.0 ** -24 * (2.0 * r - 1.0)), "v": st["v"]})
+    e.step(steps)
+    sens = []
+    for i in range(count):
+        x, y = ref[i], e.get_state(i)
+        num = sum(float(np.sum((y[f] - x[f]) ** 2)) for f in ("u", "v"))
+        den = sum(float(np.sum(x[f] ** 2)) for f in ("u", "v"))
+        sens.append((num / den) ** 0.5)
+    sens = np.array(sens)
+    out["f64_perturbed_2^-24_rel_l2"] = {
+        "median": float(np.median(sens)), "instances_le_1e-5": int((sens <= 1e-5).sum()),
+        "f32k_above_1e-5_where_f64_control_le_1e-5": int(((errs > 1e-5) & (sens <= 1e-5)).sum()),
+        "f32k_le_1e-5_where_f64_control_above_1e-5": int(((errs <= 1e-5) & (sens > 1e-5)).sum())}
     for e in ens.values():
         e.close()
     return out
