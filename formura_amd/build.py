@@ -78,6 +78,9 @@ REFERENCE_ROOT = os.environ.get("FORMURA_REFERENCE", "/root/reference")
 SAMPLES: Dict[str, tuple] = {
     "hllc1d_64": ("sample/1d_hllc/hllc.fmr", "hllc1d_64.yaml", "auto"),
     "hllc1d_400": ("sample/1d_hllc/hllc.fmr", "hllc1d_400.yaml", "auto"),
+    # the reference's own configurations, on one rank (for their exact solutions)
+    "mhd1d_3200": ("sample/mhd_1d_hlld/mhd_hlld.fmr", "mhd1d_3200.yaml", "auto"),
+    "hllc3d_shock": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_shock.yaml", "auto"),
     "mhd1d_64": ("sample/mhd_1d_hlld/mhd_hlld.fmr", "mhd1d_64.yaml", "auto"),
     "mhd3d_8": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_8.yaml", "auto"),
     "hllc3d_8": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_8.yaml", "auto"),

@@ -11,8 +11,9 @@ fixture is written the run is checked against what the mathematics fixes:
   periodic grid keeps every conserved total (all fields but the GLM cleaning
   potential psi, which is damped) to rounding;
 * a uniform state is a fixed point, bit for bit (equal fluxes cancel exactly);
-* the shock tube of sample/1d_hllc, run from the program's own init graph at
-  the reference's configuration, stays within a bounded L1 distance of the
+* the shock tubes of sample/1d_hllc, sample/3d_hllc (MUSCL-Hancock, along y)
+  and sample/mhd_1d_hlld (Ryu-Jones 2a, HLLD), each run from the program's own
+  init graph at the reference's configuration, stays within a bounded L1 distance of the
   reference's exact Riemann solution (sample/1d_hllc/ref_data/exact.dat) in
   the tube's interior, and the drifting frame's cell map gives a smaller
   distance than the same map shifted by one or two cells.
@@ -112,6 +113,78 @@ def l1_vs_exact(final, steps, exact, shift=0):
     return float(np.mean(np.abs(rho - exact[:, 1])[150:350]))
 
 
+# The reference's own configurations (one rank over the whole grid) run from the
+# program's init graph and compared with the exact solutions the reference ships:
+# name -> (steps, exact-solution file, [(field, column of the file)], cells the
+# cells compared (first, count), L1 bound per field, cell of the file's first row)
+EXACT_CASES = {
+    "mhd1d_3200": (1600, "sample/mhd_1d_hlld/ref_data/Bx2.dat", [("rho", 3), ("B_y", 9)], (1200, 800), 0.002, 1200),
+    "hllc3d_shock": (100, "sample/1d_hllc/ref_data/exact.dat", [("rho", 1)], (150, 200), 0.01, 0),
+}
+
+
+def field_hashes(st):
+    import hashlib
+    return {f: hashlib.sha256(np.ascontiguousarray(st[f], dtype=np.float64).tobytes()).hexdigest() for f in sorted(st)}
+
+
+def profile(name, st):
+    """The first axis' profile of a 1-D run, the y profile (x = z = 0) of the 3-D tube."""
+    return {f: (v if v.ndim == 1 else v[0, :, 0]) for f, v in st.items()}
+
+
+def exact_l1(name, prof, steps, exact_rows, shift=0):
+    """Mean |f - f_exact| over the cells the exact solution covers, per compared field.
+    Output index i holds cell (i + offset) mod N with offset = -2*steps (sleeve 2)."""
+    _, _, cols, (first, count), _, row0 = EXACT_CASES[name]
+    n = len(next(iter(prof.values())))
+    k = (np.arange(n) + (-2 * steps) % n + shift) % n
+    out = {}
+    for f, c in cols:
+        v = np.empty(n)
+        v[k] = prof[f]
+        out[f] = float(np.mean(np.abs(v[first:first + count] - exact_rows[first - row0:first - row0 + count, c])))
+    return out
+
+
+def exact_rows_of(name):
+    path = os.path.join(B.REFERENCE_ROOT, EXACT_CASES[name][1])
+    ex = np.loadtxt(path)
+    return ex[::10] if name == "hllc3d_shock" else ex
+
+
+def main_exact():
+    for name, (steps, rel, cols, (first, count), bound, row0) in EXACT_CASES.items():
+        mm, desc = front_end(name)
+        st = run(mm, desc, mm_interp.init(mm.init, tuple(desc["grid"])), steps)
+        if len(desc["grid"]) == 3:
+            for f, v in st.items():
+                assert (v == v[:1, :, :1]).all(), (name, f, "not uniform across x and z")
+        ex = exact_rows_of(name)
+        keep = [0] + [c for _, c in cols]
+        np.savetxt(os.path.join(HERE, "exact_%s.txt" % name), ex[:, keep], fmt="%.9e",
+                   header="%s of formura v2.3.2, columns x %s%s" % (
+                       rel, " ".join(f for f, _ in cols), " (every 10th line)" if name == "hllc3d_shock" else ""))
+        prof = profile(name, st)
+        ref = np.loadtxt(os.path.join(HERE, "exact_%s.txt" % name))
+        ref_full = np.zeros((ref.shape[0], max(c for _, c in cols) + 1))
+        for j, (_, c) in enumerate(cols):
+            ref_full[:, c] = ref[:, j + 1]
+        err = exact_l1(name, prof, steps, ref_full)
+        shifted = {sh: exact_l1(name, prof, steps, ref_full, sh) for sh in (-2, -1, 1, 2)}
+        for f in err:
+            assert err[f] < bound and all(err[f] < e[f] for e in shifted.values()), (name, f, err, shifted)
+        meta = {"program": name, "source": B.SAMPLES[name][0], "seed": None, "steps": steps, "grid": desc["grid"],
+                "offset": [(-desc["sleeve"] * steps) % n for n in desc["grid"]], "sha256": field_hashes(st),
+                "exact": rel, "exact_columns": cols, "exact_cells": [first, count], "l1_bound": bound,
+                "checked": "mean |f - f_exact| %s over cells %d..%d; frame shifts -2..2 give %s" % (
+                    ", ".join("%s %.5f" % kv for kv in err.items()), first, first + count - 1,
+                    "; ".join("%+d: %s" % (sh, ", ".join("%.5f" % v for v in e.values())) for sh, e in shifted.items()))}
+        with open(os.path.join(HERE, "sample_%s.json" % name), "w") as fh:
+            json.dump(meta, fh, indent=1)
+        print("wrote", name, meta["checked"], flush=True)
+
+
 def main():
     if not B.reference_present():
         sys.exit("formura checkout not found at %s (set FORMURA_REFERENCE)" % B.REFERENCE_ROOT)
@@ -143,3 +216,4 @@ def main():
 
 if __name__ == "__main__":
     main()
+    main_exact()

@@ -151,3 +151,80 @@ def test_mixed_sample_against_fp64(name):
     cons = [f for f in t0 if f != "psi"]
     scale = max(abs(t0[f]) for f in cons)
     assert max(abs(t1[f] - t0[f]) for f in cons) <= 1e-13 * scale
+
+
+EXACT = ["mhd1d_3200", "hllc3d_shock"]
+
+
+def _exact_l1(meta, name, fields_frame):
+    """Mean |f - f_exact| over the compared cells (the fixture's exact_<name>.txt
+    holds x and the compared columns; output index i is cell (i + offset) mod N)."""
+    ex = np.loadtxt(os.path.join(GOLDEN, "exact_%s.txt" % name))
+    first, count = meta["exact_cells"]
+    row0 = first if ex.shape[0] == count else 0
+    out = {}
+    for j, (f, _) in enumerate(meta["exact_columns"]):
+        axis = 0 if fields_frame[f].ndim == 1 else 1  # the 3-D tube runs along y
+        prof = fields_frame[f] if axis == 0 else fields_frame[f][0, :, 0]
+        n = prof.shape[0]
+        v = np.empty(n)
+        v[(np.arange(n) + meta["offset"][axis]) % n] = prof
+        out[f] = float(np.mean(np.abs(v[first:first + count] - ex[first - row0:first - row0 + count, j + 1])))
+    return out
+
+
+@pytest.mark.skipif(not B.reference_present(), reason="formura checkout not present")
+@pytest.mark.parametrize("name", EXACT)
+def test_interpreter_reproduces_reference_configuration(name):
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import make_samples as M
+    import mm_interp
+    with open(os.path.join(GOLDEN, "sample_%s.json" % name)) as f:
+        meta = json.load(f)
+    mm, desc = M.front_end(name)
+    st = M.run(mm, desc, mm_interp.init(mm.init, tuple(desc["grid"])), meta["steps"])
+    assert M.field_hashes(st) == meta["sha256"]
+
+
+def test_exact_solution_fixtures_pinned():
+    """The stored exact solutions and the bounds agree with the fixtures' own record."""
+    for name in EXACT:
+        with open(os.path.join(GOLDEN, "sample_%s.json" % name)) as f:
+            meta = json.load(f)
+        ex = np.loadtxt(os.path.join(GOLDEN, "exact_%s.txt" % name))
+        assert ex.shape[1] == 1 + len(meta["exact_columns"]) and meta["l1_bound"] <= 0.01
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", EXACT)
+def test_reference_configuration_against_exact_solution(name):
+    """The HLLD MHD (Ryu-Jones 2a) and MUSCL-Hancock HLLC shock tubes at the
+    reference's configurations, from the programs' own init graphs: the GPU state
+    equals the interpreter's bit for bit (SHA-256 per field), Navi included, the
+    3-D tube stays uniform across x and z, and the profile is within the
+    fixture's bound of the reference's exact solution."""
+    import hashlib
+    path = B.lib_path(name)
+    if not os.path.exists(path):
+        pytest.skip("%s is built only where the formura checkout is present" % name)
+    with open(os.path.join(GOLDEN, "sample_%s.json" % name)) as f:
+        meta = json.load(f)
+    print("reference configuration %s: %d steps" % (name, meta["steps"]), flush=True)
+    lib = load_lib(path)
+    lib.init()
+    lib.set_resident(True)
+    lib.upload()
+    lib.forward_steps(meta["steps"])
+    lib.download()
+    lib.set_resident(False)
+    nv = lib.navi_dict()
+    assert [nv["offset_" + a] for a in lib.axes] == meta["offset"]
+    st = {f: lib.field(f).copy() for f in lib.fields}
+    got = {f: hashlib.sha256(np.ascontiguousarray(st[f], dtype=np.float64).tobytes()).hexdigest() for f in sorted(st)}
+    assert got == meta["sha256"]
+    if len(meta["grid"]) == 3:
+        for f, v in st.items():
+            assert (v == v[:1, :, :1]).all(), f
+    for f, e in _exact_l1(meta, name, st).items():
+        assert e < meta["l1_bound"], (f, e)
