@@ -57,6 +57,7 @@ PROGRAMS: Dict[str, tuple] = {
     "gs_64_tile_k8": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 8}),
     "gs_64_tile_k5_inplace": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 5, "inplace": 1}),
     "gs_64_wave_k3": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "wave2d", "rows": 16, "k": 3}),
+    "gs_64_tile_t128": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 3, "threads": 128}),
     "gs_2048_wave": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "wave2d", "rows": 32, "k": 3}),
     # mixed precision: FP32 flux arithmetic, FP64 wave speeds, positivity checks and update
     "euler3d_16_mixed": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),
@@ -203,7 +204,7 @@ def _tuned_batch(name: str):
 # ensemble (64..1024 in steps of 32): the per-instance FP64 GPU path that the
 # batched ensemble is compared with.  Schedule: the one tuned for 2048^2.
 ENSEMBLE_SIDES = tuple(range(64, 1025, 32))
-SIDE_SPEC = {"kind": "tile2d", "tx": 16, "ty": 32, "k": 3}
+SIDE_SPEC = {"kind": "tile2d", "tx": 32, "ty": 32, "k": 4, "threads": 512}
 
 
 def side_name(n: int) -> str:

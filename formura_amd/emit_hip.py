@@ -445,8 +445,10 @@ class Tile2DSchedule:
 
     name = "tile2d"
 
-    def __init__(self, tx, ty, k, inplace=False):
+    def __init__(self, tx, ty, k, inplace=False, threads=256):
         self.tx, self.ty, self.k = tx, ty, k
+        # work-items per workgroup (a level's points are dealt round-robin)
+        self.threads = int(threads)
         # in place: one LDS buffer per field; a level's values go to registers
         # and overwrite the previous level after a barrier (half the LDS, so
         # more workgroups per CU, for one more barrier per level)
@@ -489,7 +491,7 @@ class Tile2DSchedule:
                                           "const long long g = (long long)%s(x0 - H + r) * N2 + %s(y0 - H + c);"
                                           % (wrap1, wrap2)],
                               [("B%d[0][idx]" % i, "FLD%d(st_in)[g]" % i) for i in range(len(fields))],
-                              fields[0][1], "ld")
+                              fields[0][1], "ld", threads=self.threads)
         srcs = ", ".join("B%d[src]" % i for i in range(len(fields)))
         decl_o = " ".join("%s o%d;" % (ct, i) for i, (n, ct) in enumerate(fields))
         refs_o = ", ".join("o%d" % i for i in range(len(fields)))
@@ -497,7 +499,7 @@ class Tile2DSchedule:
         st_glb = " ".join("FLD%d(st_out)[g] = o%d;" % (i, i) for i in range(len(fields)))
         if self.inplace:
             out.append(TILE2D_INPLACE % dict(
-                TX=TX, TY=TY, lds=lds, load_tile=load_tile, decl_o=decl_o, refs_o=refs_o,
+                TX=TX, TY=TY, NTH=self.threads, lds=lds, load_tile=load_tile, decl_o=decl_o, refs_o=refs_o,
                 srcs=", ".join("B%d[0]" % i for i in range(len(fields))),
                 rdecl=" ".join("%s res%d[NI];" % (ct, i) for i, (n, ct) in enumerate(fields)),
                 rset=" ".join("res%d[it] = o%d;" % (i, i) for i in range(len(fields))),
@@ -506,7 +508,7 @@ class Tile2DSchedule:
             return out
         out.append("""
 template <int K>
-__global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
+__global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
   constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
   constexpr int NTX = N1 / TX, NTY = N2 / TY, NT = NTX * NTY;
 %(lds)s
@@ -521,7 +523,7 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
   for (int j = 1; j <= K; ++j) {
     const int src = (j - 1) & 1, dst = j & 1;
     const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
-    for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
+    for (int idx = threadIdx.x; idx < HH * W; idx += %(NTH)d) {
       const int rr = idx / W;
       const int r = lo + rr, c = lo + (idx - rr * W);
       const int o = r * RY + c;
@@ -537,7 +539,7 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
     if (j < K) __syncthreads();
   }
 }
-""" % dict(TX=TX, TY=TY, lds=lds, wrap1=wrap1, wrap2=wrap2, load_tile=load_tile, decl_o=decl_o,
+""" % dict(TX=TX, TY=TY, NTH=self.threads, lds=lds, wrap1=wrap1, wrap2=wrap2, load_tile=load_tile, decl_o=decl_o,
            srcs=srcs, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb))
         return out
 
@@ -546,8 +548,8 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
         return ("static void run_step_tile(int k, const char* in, char* out, hipStream_t s) {\n"
                 "  switch (k) {\n%s  default: fmr_rt::fatal(\"bad step count\");\n  }\n"
                 "  FMR_CHECK(hipGetLastError());\n}\n"
-                % "".join("  case %d: hipLaunchKernelGGL(k_step_tile<%d>, dim3(%d), dim3(256), 0, s, in, out); break;\n"
-                          % (k, k, nt) for k in range(1, self.k + 1)))
+                % "".join("  case %d: hipLaunchKernelGGL(k_step_tile<%d>, dim3(%d), dim3(%d), 0, s, in, out); break;\n"
+                          % (k, k, nt, self.threads) for k in range(1, self.k + 1)))
 
     def advance(self, em):
         return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
@@ -560,7 +562,7 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
 
 TILE2D_INPLACE = """
 template <int K>
-__global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
+__global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
   constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
   constexpr int NTX = N1 / TX, NTY = N2 / TY, NT = NTX * NTY;
 %(lds)s
@@ -572,14 +574,14 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
 %(load_tile)s
   __syncthreads();
   // points of the largest level per thread; level j covers [2js, RX) x [2js, RY)
-  constexpr int NI = ((RX - 2 * SLEEVE) * (RY - 2 * SLEEVE) + 255) / 256;
+  constexpr int NI = ((RX - 2 * SLEEVE) * (RY - 2 * SLEEVE) + %(NTH)d - 1) / %(NTH)d;
 #pragma unroll
   for (int j = 1; j <= K; ++j) {
     const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
     %(rdecl)s
 #pragma unroll
     for (int it = 0; it < NI; ++it) {
-      const int idx = threadIdx.x + it * 256;
+      const int idx = threadIdx.x + it * %(NTH)d;
       if (idx < HH * W) {
         const int rr = idx / W;
         const int r = lo + rr, c = lo + (idx - rr * W);
@@ -598,7 +600,7 @@ __global__ void __launch_bounds__(256) k_step_tile(const char* __restrict__ st_i
       __syncthreads();  // every read of level j-1 is done
 #pragma unroll
       for (int it = 0; it < NI; ++it) {
-        const int idx = threadIdx.x + it * 256;
+        const int idx = threadIdx.x + it * %(NTH)d;
         if (idx < HH * W) {
           const int rr = idx / W;
           const int o = (lo + rr) * RY + lo + (idx - rr * W);
@@ -957,7 +959,7 @@ def _stream3d_ok(em, ty, tz, cx, k):
 
 def _wave2d_ok(em, rows, k):
     plan = em.step
-    if em.dim != 2 or plan.temps or em.cfg.blocking is not None and False:
+    if em.dim != 2 or plan.temps:
         return False
     if any(m.inst[0] == "LoadIndex" for kk in plan.stores for m in plan.mm[kk].micro):
         return False
@@ -986,7 +988,7 @@ def schedule_from_spec(em, spec: dict):
         args = (int(spec["tx"]), int(spec["ty"]), int(spec["k"]))
         if not _tile2d_ok(em, *args):
             raise EmitError("tile2d %r does not apply to %s" % (spec, em.name))
-        return Tile2DSchedule(*args, inplace=bool(spec.get("inplace", 0)))
+        return Tile2DSchedule(*args, inplace=bool(spec.get("inplace", 0)), threads=int(spec.get("threads", 256)))
     if kind == "wave2d":
         args = (int(spec["rows"]), int(spec["k"]))
         if not _wave2d_ok(em, *args):
@@ -1024,6 +1026,13 @@ def candidate_specs(em) -> List[dict]:
                     out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k})
                     if k > 1:
                         out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k, "inplace": 1})
+                    if k in (2, 3, 4, 6):
+                        for nth in (128, 512, 1024):
+                            if nth == 128 and tx * ty > 512:
+                                continue
+                            out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k, "threads": nth})
+                            if nth > 256:
+                                out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k, "threads": nth, "inplace": 1})
         for rows in (8, 16, 32, 64):
             for k in (1, 2, 3, 4):
                 if _wave2d_ok(em, rows, k):
