@@ -143,6 +143,9 @@ def batch_candidates(precision: str):
                 continue
             for grid in (1024, 2048, 4096):
                 out.append({"tx": tx, "ty": ty, "k": k, "grid": grid})
+            if tx * ty >= 512 and k in (3, 4, 5):
+                for grid in (1024, 2048):
+                    out.append({"tx": tx, "ty": ty, "k": k, "grid": grid, "threads": 512})
     return out
 
 

@@ -144,12 +144,13 @@ def build_batch(name: str, force: bool = False, spec: Optional[dict] = None, out
     fmr, yml, prec, params, tx, ty, k = BATCHES[name]
     if spec is None:
         spec = _tuned_batch(name)
-    grid = 2048
+    grid, threads = 2048, 256
     if spec:
         tx, ty, k = int(spec["tx"]), int(spec["ty"]), int(spec["k"])
         grid = int(spec.get("grid", grid))
+        threads = int(spec.get("threads", threads))
     _, _, mm = generate(os.path.join(PROG_DIR, fmr), os.path.join(PROG_DIR, yml), name, "flat")
-    src = BatchEmitter(mm, name, prec, params, tx, ty, k, grid_wgs=grid).emit()
+    src = BatchEmitter(mm, name, prec, params, tx, ty, k, grid_wgs=grid, threads=threads).emit()
     d = out_dir or os.path.join(BUILD_DIR, name)
     os.makedirs(d, exist_ok=True)
     p = os.path.join(d, "%s.hip" % name)

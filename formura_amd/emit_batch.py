@@ -37,7 +37,7 @@ PRECISIONS = {"f64": ("double", False), "f32k": ("float", True)}
 
 class BatchEmitter:
     def __init__(self, mm, name: str, precision: str, params: List[str], tx: int, ty: int, k: int,
-                 graph_launches: int = 16, grid_wgs: int = 2048):
+                 graph_launches: int = 16, grid_wgs: int = 2048, threads: int = 256):
         self.mm, self.om, self.name = mm, mm.om, name
         if self.om.dimension != 2:
             raise EmitError("batched ensembles are built for 2-D programs")
@@ -49,6 +49,8 @@ class BatchEmitter:
         self.tx, self.ty, self.k = tx, ty, k
         self.G = graph_launches + (graph_launches % 2)
         self.grid_wgs = grid_wgs
+        # work-items per workgroup
+        self.threads = int(threads)
         self.plan = GraphPlan(mm.step, self.om.config.sleeve, 2, "step")
         if self.plan.temps:
             raise EmitError("batched ensembles need a program without manifest temporaries")
@@ -102,7 +104,8 @@ class BatchEmitter:
             pairs += [("C%d[0][idx]" % i, "src[%dLL * plane + g]" % (nf + i)) for i in range(nf)]
         load_tile = tile_load("RX * RY", ["const int r = idx / RY, c = idx - r * RY;",
                                           "const long long g = (long long)wrapn(tx0 - H + r, I.n1) * I.n2 "
-                                          "+ wrapn(ty0 - H + c, I.n2);"], pairs, real, "ld", indent="    ")
+                                          "+ wrapn(ty0 - H + c, I.n2);"], pairs, real, "ld", indent="    ",
+                              threads=self.threads)
         srcs = ", ".join("B%d[sb]" % i for i in range(nf))
         decl = " ".join("%s old%d, inc%d;" % (real, i, i) for i in range(nf))
         refs = ", ".join("old%d, inc%d" % (i, i) for i in range(nf))
@@ -123,18 +126,18 @@ class BatchEmitter:
         desc = {"name": self.name, "precision": self.precision, "real": real, "kahan": self.kahan,
                 "fields": [n for n, _ in self.fields], "params": self.params, "sleeve": s,
                 "tile": [self.tx, self.ty], "steps_per_launch": self.k, "graph_launches": self.G,
-                "grid_wgs": self.grid_wgs, "slots_per_cell": nslots}
+                "grid_wgs": self.grid_wgs, "threads": self.threads, "slots_per_cell": nslots}
         return TEMPLATE % dict(
             name=self.name, real=real, nf=nf, nslots=nslots, np=max(1, len(self.params)), sleeve=s,
-            TX=self.tx, TY=self.ty, KMAX=self.k, G=self.G, GRID=self.grid_wgs, desc=json.dumps(desc),
+            TX=self.tx, TY=self.ty, NTH=self.threads, KMAX=self.k, G=self.G, GRID=self.grid_wgs, desc=json.dumps(desc),
             pt_params=params, pt_outs=outs, pt_body="\n".join("  " + l for l in lines),
             lds=lds, load_tile=load_tile, srcs=srcs, decl=decl, refs=refs, upd=upd, st_lds=st_lds,
             st_glb=st_glb, CB=cbase, kahan=1 if self.kahan else 0,
             preload="\n".join("  { hipFuncAttributes fa; FMR_CHECK(hipFuncGetAttributes(&fa, (const void*)k_batch<%d>)); }"
                               % kk for kk in range(1, self.k + 1)),
             launch_cases="".join(
-                "    case %d: hipLaunchKernelGGL(k_batch<%d>, dim3(grid), dim3(256), 0, b->stream, b->inst_d, "
-                "b->tile_inst_d, b->ntiles, in, out); break;\n" % (kk, kk) for kk in range(1, self.k + 1)))
+                "    case %d: hipLaunchKernelGGL(k_batch<%d>, dim3(grid), dim3(%d), 0, b->stream, b->inst_d, "
+                "b->tile_inst_d, b->ntiles, in, out); break;\n" % (kk, kk, self.threads) for kk in range(1, self.k + 1)))
 
     def _update_ids(self, node, fname, base):
         st = node.micro[-1]
@@ -178,7 +181,7 @@ __device__ __forceinline__ void pt(%(pt_params)s, const int o, const double* P, 
 __device__ __forceinline__ int wrapn(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 template <int K>
-__global__ void __launch_bounds__(256) k_batch(const Inst* __restrict__ inst, const int* __restrict__ tile_inst,
+__global__ void __launch_bounds__(%(NTH)d) k_batch(const Inst* __restrict__ inst, const int* __restrict__ tile_inst,
                                                int ntiles, const real_t* __restrict__ in, real_t* __restrict__ out) {
   constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
   constexpr int CB = %(CB)s;  // the cell's own value relative to the output point
@@ -199,7 +202,7 @@ __global__ void __launch_bounds__(256) k_batch(const Inst* __restrict__ inst, co
     for (int j = 1; j <= K; ++j) {
       const int sb = (j - 1) & 1, db = j & 1;
       const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
-      for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
+      for (int idx = threadIdx.x; idx < HH * W; idx += %(NTH)d) {
         const int rr = idx / W;
         const int r = lo + rr, c = lo + (idx - rr * W);
         const int o = r * RY + c;
