@@ -247,8 +247,8 @@ def run_adjoint(nsteps: int = 2000, snapshots: int = 32, directions: int = 16, e
         xd = torch.tensor(x0, device=dev, dtype=dt)
         with torch.no_grad():
             vt = A.run(s, xd, torch.tensor([0.035, 0.065, 2e-5, 1e-5], dtype=torch.float64), nsteps, 0)[1].clone()
-        rec = {}
-        for rep in range(2):  # the second repetition is timed warm
+        fwd, bwd = [], []
+        for rep in range(4):  # one warm-up, then the best of three timed repetitions
             x = xd.clone().requires_grad_(True)
             p = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
             torch.cuda.synchronize()
@@ -260,9 +260,12 @@ def run_adjoint(nsteps: int = 2000, snapshots: int = 32, directions: int = 16, e
             loss.backward()
             e3 = ev()
             torch.cuda.synchronize()
-            rec = {"forward_ms": e0.elapsed_time(e1), "backward_ms": e2.elapsed_time(e3),
-                   "ratio": e2.elapsed_time(e3) / e0.elapsed_time(e1), "stats": dict(s.last_stats),
-                   "unit_steps": s.levels}
+            if rep > 0:
+                fwd.append(e0.elapsed_time(e1))
+                bwd.append(e2.elapsed_time(e3))
+            del y, loss
+        rec = {"forward_ms": min(fwd), "backward_ms": min(bwd), "ratio": min(bwd) / min(fwd),
+               "forward_ms_all": fwd, "backward_ms_all": bwd, "stats": dict(s.last_stats), "unit_steps": s.levels}
         grads[name] = (x.grad.double().cpu().numpy(), p.grad.cpu().numpy(), s, vt)
         out[name] = rec
     gx, gp, s, vt = grads["gs_1024_adj"]

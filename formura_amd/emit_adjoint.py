@@ -148,7 +148,7 @@ class ReverseSweep:
 
 class AdjointEmitter:
     def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256,
-                 tile=(16, 32, 4), adj_tile=None, levels=4, row_block=1):
+                 tile=(32, 32, 4), adj_tile=(16, 64), levels=4, row_block=1, fwd_threads=512, adj_threads=512):
         self.mm, self.om, self.name = mm, mm.om, name
         self.real = real
         self.tile = tuple(tile) if tile else None
@@ -158,6 +158,8 @@ class AdjointEmitter:
         self.levels = max(1, int(levels))
         # points per work-item along the first axis in the adjoint levels kernel
         self.row_block = int(row_block)
+        # work-items per workgroup of the tiled forward and adjoint kernels
+        self.fwd_threads, self.adj_threads = int(fwd_threads), int(adj_threads)
         self.dim = self.om.dimension
         self.N = self.om.config.grid_per_node
         self.params = list(params)
@@ -431,14 +433,14 @@ class AdjointEmitter:
         lds = "\n".join("  __shared__ real_t B%d[2][RX * RY];" % i for i in range(nf))
         load_tile = tile_load("RX * RY", ["const int r = idx / RY, c = idx - r * RY;",
                                           "const long long g = (long long)wrapn(x0 - H + r, N1) * N2 + wrapn(y0 - H + c, N2);"],
-                              [("B%d[0][idx]" % i, "FLD%d(st_in)[g]" % i) for i in range(nf)], "real_t", "ld")
+                              [("B%d[0][idx]" % i, "FLD%d(st_in)[g]" % i) for i in range(nf)], "real_t", "ld", threads=self.fwd_threads)
         srcs = ", ".join("B%d[sb]" % i for i in range(nf))
         decl = " ".join("real_t o%d;" % i for i in range(nf))
         refs = ", ".join("o%d" % i for i in range(nf))
         st_lds = " ".join("B%d[db][o] = o%d;" % (i, i) for i in range(nf))
         st_lvl = " ".join("FLD%d(lvl)[g] = o%d;" % (i, i) for i in range(nf))
         st_glb = " ".join("FLD%d(dst)[g] = o%d;" % (i, i) for i in range(nf))
-        src.append(TILE_FWD % dict(TX=tx, TY=ty, lds=lds, load_tile=load_tile, srcs=srcs, decl=decl, refs=refs,
+        src.append(TILE_FWD % dict(TX=tx, TY=ty, FTH=self.fwd_threads, lds=lds, load_tile=load_tile, srcs=srcs, decl=decl, refs=refs,
                                    st_lds=st_lds, st_lvl=st_lvl, st_glb=st_glb))
         # adjoint point function on LDS tiles
         pb, _ = self.point_body(lambda f, a: "X%d[xo + (%d)*XW + (%d)]" % (f, a[0], a[1]),
@@ -456,10 +458,11 @@ class AdjointEmitter:
         xl = "\n".join("  __shared__ real_t X%d[XH * XW];" % i for i in range(nf))
         ll = "\n".join("  __shared__ real_t L%d[LH * LW];" % i for i in range(nf))
         src.append(TILE_ADJ % dict(
-            TX=ax, TY=ay, xl=xl, ll=ll, uses_state="true" if uses_state else "false",
+            TX=ax, TY=ay, ATH=self.adj_threads, xl=xl, ll=ll, uses_state="true" if uses_state else "false",
             lload=tile_load("LH * LW", ["const int r = idx / LW, c = idx - r * LW;",
                                         "const long long g = (long long)wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);"],
-                            [("L%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ll"),
+                            [("L%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ll",
+                            threads=self.adj_threads),
             xdecl=" ".join("real_t nx%d[NX_IT];" % i for i in range(nf)),
             xfetch="\n".join("          nx%d[it] = FLD%d(xs)[g];" % (i, i) for i in range(nf)),
             xput="\n".join("        X%d[idx] = nx%d[it];" % (i, i) for i in range(nf)),
@@ -474,12 +477,12 @@ class AdjointEmitter:
             pdecl=" ".join("double pgs%d = 0;" % p for p in range(np_)),
             pacc=" ".join("pgs%d += q%d;" % (p, p) for p in range(np_)),
             pgs=", ".join("pgs%d" % p for p in range(np_))))
-        fcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, false>), dim3(NT), dim3(256), 0, s, in, out, P); break;\n"
-                         % (k, k) for k in range(1, kmax + 1))
-        lcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, true>), dim3(NT), dim3(256), 0, s, in, out, P); break;\n"
-                         % (k, k) for k in range(1, self.levels))
-        acases = "".join("    case %d: hipLaunchKernelGGL(k_adjoint_levels<%d>, dim3(NTA), dim3(256), 0, s, st0, lv, lam_next, "
-                         "lam_prev, P, pgrad); break;\n" % (k, k) for k in range(1, self.levels + 1))
+        fcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, false>), dim3(NT), dim3(%d), 0, s, in, out, P); break;\n"
+                         % (k, k, self.fwd_threads) for k in range(1, kmax + 1))
+        lcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, true>), dim3(NT), dim3(%d), 0, s, in, out, P); break;\n"
+                         % (k, k, self.fwd_threads) for k in range(1, self.levels))
+        acases = "".join("    case %d: hipLaunchKernelGGL(k_adjoint_levels<%d>, dim3(NTA), dim3(%d), 0, s, st0, lv, lam_next, "
+                         "lam_prev, P, pgrad); break;\n" % (k, k, self.adj_threads) for k in range(1, self.levels + 1))
         src.append(TILE_HOST % dict(TX=tx, TY=ty, AX=ax, AY=ay, K=kmax, fcases=fcases, lcases=lcases, acases=acases))
         return "\n".join(src)
 
@@ -491,7 +494,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) { return (nt %% 8 == 0) ?
 __device__ __forceinline__ int wrapn(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 template <int K, bool ALL>
-__global__ void __launch_bounds__(256) k_forward_tile(const char* __restrict__ st_in, char* __restrict__ st_out,
+__global__ void __launch_bounds__(%(FTH)d) k_forward_tile(const char* __restrict__ st_in, char* __restrict__ st_out,
                                                       const double* __restrict__ P) {
   constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
   constexpr int NTY = N2 / TY, NT = (N1 / TX) * NTY;
@@ -510,7 +513,7 @@ __global__ void __launch_bounds__(256) k_forward_tile(const char* __restrict__ s
     const int sb = (j - 1) & 1, db = j & 1;
     const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
     char* __restrict__ dst = ALL ? st_out + (size_t)(j - 1) * STATE_BYTES : st_out;
-    for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
+    for (int idx = threadIdx.x; idx < HH * W; idx += %(FTH)d) {
       const int rr = idx / W;
       const int r = lo + rr, c = lo + (idx - rr * W);
       const int o = r * RY + c;
@@ -536,7 +539,7 @@ __global__ void __launch_bounds__(256) k_forward_tile(const char* __restrict__ s
 TILE_ADJ = r"""
 // K adjoint steps: lam_K (HBM) -> lam_0 (HBM); X_0 = st0, X_m = level buffer m-1 of lv
 template <int K>
-__global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__ st0, const char* __restrict__ lv,
+__global__ void __launch_bounds__(%(ATH)d) k_adjoint_levels(const char* __restrict__ st0, const char* __restrict__ lv,
                                                         const char* __restrict__ lam_next, char* __restrict__ lam_prev,
                                                         const double* __restrict__ P, double* __restrict__ pgrad) {
   constexpr int TX = %(TX)d, TY = %(TY)d;
@@ -558,13 +561,13 @@ __global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__
   // Element it of this thread is (r, c) = divmod(idx, XW) of the largest region;
   // its periodic global offset xg and the lowest level that reads it (level m
   // reads rows < TX + 2(m+2)s and columns < TY + 2(m+2)s) are computed once.
-  constexpr int NX_IT = (XH * XW + 255) / 256;
+  constexpr int NX_IT = (XH * XW + %(ATH)d - 1) / %(ATH)d;
   static_assert((long long)N1 * N2 < (1LL << 31), "element offsets of a field are ints");
   %(xdecl)s
   int xg[NX_IT], xm[NX_IT];
 #pragma unroll
   for (int it = 0; it < NX_IT; ++it) {
-    const int idx = threadIdx.x + it * 256;
+    const int idx = threadIdx.x + it * %(ATH)d;
     const int r = idx / XW, c = idx - r * XW;
     xg[it] = wrapn(x0 - 2 * SLEEVE + r, N1) * N2 + wrapn(y0 - 2 * SLEEVE + c, N2);
     const int er = r - TX, ec = c - TY, e = er > ec ? er : ec;  // beyond the tile by e cells
@@ -583,7 +586,7 @@ __global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__
   auto xput = [&](int m) {
 #pragma unroll
     for (int it = 0; it < NX_IT; ++it) {
-      const int idx = threadIdx.x + it * 256;
+      const int idx = threadIdx.x + it * %(ATH)d;
       if (xm[it] <= m) {
 %(xput)s
       }
@@ -603,7 +606,7 @@ __global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__
   // LDS reads that their stencils share are issued once
   constexpr int RB = %(rb)d;
   static_assert(TX %% RB == 0 && (2 * SLEEVE) %% RB == 0, "row blocks tile every level");
-  constexpr int NL_IT = ((LH / RB) * LW + 255) / 256;
+  constexpr int NL_IT = ((LH / RB) * LW + %(ATH)d - 1) / %(ATH)d;
 #pragma unroll
   for (int mm = 0; mm < K; ++mm) {
     const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} -> lam_m at state X_m
@@ -613,7 +616,7 @@ __global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__
     for (int it = 0; it < NL_IT; ++it) {
       // whole rows of the adjoint buffer, columns beyond the level masked: a
       // wavefront's LDS accesses are contiguous (no bank conflicts at row wraps)
-      const int idx = threadIdx.x + it * 256;
+      const int idx = threadIdx.x + it * %(ATH)d;
       const int rq = idx / LW, c = idx - rq * LW;
       if (idx < (HH / RB) * LW && c < W) {
 #pragma unroll
@@ -636,7 +639,7 @@ __global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__
       __syncthreads();  // every read of lam_{m+1} and X_m is done
 #pragma unroll
       for (int it = 0; it < NL_IT; ++it) {
-        const int idx = threadIdx.x + it * 256;
+        const int idx = threadIdx.x + it * %(ATH)d;
         const int rq = idx / LW, c = idx - rq * LW;
         if (idx < (HH / RB) * LW && c < W) {
 #pragma unroll
@@ -653,7 +656,7 @@ __global__ void __launch_bounds__(256) k_adjoint_levels(const char* __restrict__
       __syncthreads();
     }
   }
-  { const double pv[NP] = {%(pgs)s}; block_reduce_params<256>(pgrad + (long long)blockIdx.x * NP, pv); }
+  { const double pv[NP] = {%(pgs)s}; block_reduce_params<%(ATH)d>(pgrad + (long long)blockIdx.x * NP, pv); }
 }
 """
 
