@@ -50,6 +50,7 @@ PROGRAMS: Dict[str, tuple] = {
     # fixed schedules, so that every schedule is parity-tested at small sizes
     "heat3d_32_stream": ("heat3d.fmr", "heat3d_32.yaml", {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 2}),
     "heat3d_64_tb_stream": ("heat3d.fmr", "heat3d_64_tb.yaml", {"kind": "stream3d", "ty": 4, "tz": 16, "cx": 16, "k": 3}),
+    "heat3d_32_stream512": ("heat3d.fmr", "heat3d_32.yaml", {"kind": "stream3d", "ty": 16, "tz": 32, "cx": 8, "k": 2, "threads": 512}),
     "gs_64_flat": ("grayscott2d.fmr", "gs_64.yaml", "flat"),
     "euler3d_16_inline": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1}),
     "euler3d_32_inline_stream": ("euler3d.fmr", "euler3d_32.yaml",

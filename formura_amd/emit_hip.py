@@ -815,8 +815,10 @@ class Stream3DSchedule:
 
     name = "stream3d"
 
-    def __init__(self, ty, tz, cx, k):
+    def __init__(self, ty, tz, cx, k, threads=256):
         self.ty, self.tz, self.cx, self.k = ty, tz, cx, k
+        # work-items per workgroup
+        self.threads = int(threads)
 
     def kernels(self, em, plan):
         s = plan.sleeve
@@ -858,7 +860,7 @@ class Stream3DSchedule:
         st_glb = " ".join("FLD%d(st_out)[g] = o%d;" % (i, i) for i in range(nf))
         out.append("""
 template <int K>
-__global__ void __launch_bounds__(256) k_step_stream(const char* __restrict__ st_in, char* __restrict__ st_out) {
+__global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict__ st_in, char* __restrict__ st_out) {
   constexpr int TY = %(TY)d, TZ = %(TZ)d, CX = %(CX)d, R = %(R)d;
   constexpr int H = 2 * K * SLEEVE, RY = TY + H, RZ = TZ + H, PLANE = RY * RZ;
   constexpr int NTY = N2 / TY, NTZ = N3 / TZ, NCX = N1 / CX, NT = NTY * NTZ * NCX;
@@ -868,13 +870,13 @@ __global__ void __launch_bounds__(256) k_step_stream(const char* __restrict__ st
   const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
   const int cz = tile %% NTZ, cy = (tile / NTZ) %% NTY, cxi = tile / (NTZ * NTY);
   const int x0 = cxi * CX, y0 = cy * TY, z0 = cz * TZ;
-  constexpr int PIT = (PLANE + 255) / 256;
+  constexpr int PIT = (PLANE + %(NTH)d - 1) / %(NTH)d;
   %(pf_decl)s
   auto prefetch = [&](int t) {
     const long long gx = (long long)%(w0)s(t) * N2;
 #pragma unroll
     for (int it = 0; it < PIT; ++it) {
-      const int idx = threadIdx.x + it * 256;
+      const int idx = threadIdx.x + it * %(NTH)d;
       if (idx < PLANE) {
         const int ly = idx / RZ, lz = idx - ly * RZ;
         const long long g = (gx + %(w1)s(y0 - H + ly)) * N3 + %(w2)s(z0 - H + lz);
@@ -888,7 +890,7 @@ __global__ void __launch_bounds__(256) k_step_stream(const char* __restrict__ st
     // every read of this ring slot's previous plane ended before the last barrier
 #pragma unroll
     for (int it = 0; it < PIT; ++it) {
-      const int idx = threadIdx.x + it * 256;
+      const int idx = threadIdx.x + it * %(NTH)d;
       if (idx < PLANE) {
 %(pf_store)s
       }
@@ -902,7 +904,7 @@ __global__ void __launch_bounds__(256) k_step_stream(const char* __restrict__ st
 #pragma unroll
         for (int d = 0; d < R; ++d) pl[d] = ((((t - 2 * SLEEVE + d) %% R) + R) %% R) * PLANE;
         const int lo = 2 * j * SLEEVE, W = RZ - lo, HH = RY - lo;
-        for (int idx = threadIdx.x; idx < HH * W; idx += 256) {
+        for (int idx = threadIdx.x; idx < HH * W; idx += %(NTH)d) {
           const int rr = idx / W;
           const int ly = lo + rr, lz = lo + (idx - rr * W);
           const int o = ly * RZ + lz;
@@ -920,7 +922,7 @@ __global__ void __launch_bounds__(256) k_step_stream(const char* __restrict__ st
     }
   }
 }
-""" % dict(TY=self.ty, TZ=self.tz, CX=self.cx, R=R, lds=lds, w0=wrap[0], w1=wrap[1], w2=wrap[2],
+""" % dict(TY=self.ty, TZ=self.tz, CX=self.cx, R=R, NTH=self.threads, lds=lds, w0=wrap[0], w1=wrap[1], w2=wrap[2],
            pf_decl=pf_decl, pf_load=pf_load, pf_store=pf_store, srcs=srcs, decl_o=decl_o, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb))
         return out
 
@@ -929,8 +931,8 @@ __global__ void __launch_bounds__(256) k_step_stream(const char* __restrict__ st
         return ("static void run_step_stream(int k, const char* in, char* out, hipStream_t s) {\n"
                 "  switch (k) {\n%s  default: fmr_rt::fatal(\"bad step count\");\n  }\n"
                 "  FMR_CHECK(hipGetLastError());\n}\n"
-                % "".join("  case %d: hipLaunchKernelGGL(k_step_stream<%d>, dim3(%d), dim3(256), 0, s, in, out); break;\n"
-                          % (k, k, nt) for k in range(1, self.k + 1)))
+                % "".join("  case %d: hipLaunchKernelGGL(k_step_stream<%d>, dim3(%d), dim3(%d), 0, s, in, out); break;\n"
+                          % (k, k, nt, self.threads) for k in range(1, self.k + 1)))
 
     def advance(self, em):
         return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
@@ -998,7 +1000,7 @@ def schedule_from_spec(em, spec: dict):
         args = (int(spec["ty"]), int(spec["tz"]), int(spec["cx"]), int(spec["k"]))
         if not _stream3d_ok(em, *args):
             raise EmitError("stream3d %r does not apply to %s" % (spec, em.name))
-        return Stream3DSchedule(*args)
+        return Stream3DSchedule(*args, threads=int(spec.get("threads", 256)))
     raise EmitError("unknown schedule kind %r" % (kind,))
 
 
@@ -1045,6 +1047,8 @@ def candidate_specs(em) -> List[dict]:
                     nwg = (em.N[0] // cx) * (em.N[1] // ty) * (em.N[2] // tz) if _stream3d_ok(em, ty, tz, cx, k) else 0
                     if nwg >= 128 and (nwg <= 4096 or cx >= 32):
                         out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": k})
+                        if ty * tz >= 512 and k in (1, 2, 3) and cx >= 32:
+                            out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": k, "threads": 512})
     return out
 
 
