@@ -59,6 +59,7 @@ PROGRAMS: Dict[str, tuple] = {
     "gs_64_tile_k5_inplace": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 5, "inplace": 1}),
     "gs_64_wave_k3": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "wave2d", "rows": 16, "k": 3}),
     "gs_64_tile_t128": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 3, "threads": 128}),
+    "gs_64_tile_92k": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 64, "ty": 64, "k": 6, "inplace": 1, "threads": 1024}),
     "gs_2048_wave": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "wave2d", "rows": 32, "k": 3}),
     # mixed precision: FP32 flux arithmetic, FP64 wave speeds, positivity checks and update
     "euler3d_16_mixed": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),

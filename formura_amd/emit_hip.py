@@ -969,7 +969,7 @@ def _wave2d_ok(em, rows, k):
     return h < 48 and em.N[1] >= 64 and em.N[0] % rows == 0 and em.N[0] > h and em.N[1] > h
 
 
-def _tile2d_ok(em, tx, ty, k):
+def _tile2d_ok(em, tx, ty, k, inplace=False):
     plan = em.step
     if em.dim != 2 or plan.temps:
         return False
@@ -977,8 +977,17 @@ def _tile2d_ok(em, tx, ty, k):
         return False
     if em.N[0] % tx or em.N[1] % ty:
         return False
-    lds = sum(8 if ct == "double" else 4 for _, ct in em.fields) * 2 * (tx + 2 * k * plan.sleeve) * (ty + 2 * k * plan.sleeve)
-    return lds <= 64 * 1024
+    return _tile2d_lds(em, tx, ty, k, inplace) <= LDS_PER_WORKGROUP
+
+
+# gfx950: 160 KiB of LDS per CU, all of it available to one workgroup
+LDS_PER_WORKGROUP = 160 * 1024
+
+
+def _tile2d_lds(em, tx, ty, k, inplace=False):
+    s = em.step.sleeve
+    return (sum(8 if ct == "double" else 4 for _, ct in em.fields) * (1 if inplace else 2)
+            * (tx + 2 * k * s) * (ty + 2 * k * s))
 
 
 def schedule_from_spec(em, spec: dict):
@@ -988,7 +997,7 @@ def schedule_from_spec(em, spec: dict):
         return FlatSchedule()
     if kind == "tile2d":
         args = (int(spec["tx"]), int(spec["ty"]), int(spec["k"]))
-        if not _tile2d_ok(em, *args):
+        if not _tile2d_ok(em, *args, inplace=bool(spec.get("inplace", 0))):
             raise EmitError("tile2d %r does not apply to %s" % (spec, em.name))
         return Tile2DSchedule(*args, inplace=bool(spec.get("inplace", 0)), threads=int(spec.get("threads", 256)))
     if kind == "wave2d":
@@ -1022,19 +1031,19 @@ def candidate_specs(em) -> List[dict]:
                     out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": 1, "inline": 1})
         return out
     if em.dim == 2:
-        for tx, ty in ((16, 64), (32, 32), (32, 64), (16, 128), (8, 128), (64, 64), (64, 32), (16, 32)):
+        for tx, ty in ((16, 32), (32, 32), (16, 64), (32, 64), (64, 32), (64, 64), (8, 128), (16, 128)):
             for k in (1, 2, 3, 4, 6, 8):
-                if _tile2d_ok(em, tx, ty, k):
-                    out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k})
-                    if k > 1:
-                        out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k, "inplace": 1})
-                    if k in (2, 3, 4, 6):
-                        for nth in (128, 512, 1024):
-                            if nth == 128 and tx * ty > 512:
-                                continue
-                            out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k, "threads": nth})
-                            if nth > 256:
-                                out.append({"kind": "tile2d", "tx": tx, "ty": ty, "k": k, "threads": nth, "inplace": 1})
+                for nth in (256, 512, 1024):
+                    if nth > tx * ty:
+                        continue
+                    for inplace in ((0,) if k == 1 else (0, 1)):
+                        if _tile2d_ok(em, tx, ty, k, inplace):
+                            spec = {"kind": "tile2d", "tx": tx, "ty": ty, "k": k}
+                            if inplace:
+                                spec["inplace"] = 1
+                            if nth != 256:
+                                spec["threads"] = nth
+                            out.append(spec)
         for rows in (8, 16, 32, 64):
             for k in (1, 2, 3, 4):
                 if _wave2d_ok(em, rows, k):

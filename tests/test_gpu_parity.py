@@ -101,6 +101,7 @@ def test_resident_multi_step_launches(name, steps):
     ("gs_64_wave_k3", "gs_64", 47, True),             # register windows along x, DPP lane shifts, no LDS
     ("gs_64_tile_t128", "gs_64", 29, True),           # 128 work-items per workgroup
     ("heat3d_32_stream512", "heat3d_32", 9, True),    # stream3d with 512 work-items per workgroup
+    ("gs_64_tile_92k", "gs_64", 25, True),            # 92 KiB of LDS in one workgroup (gfx950 allows 160)
     ("gs_2048_wave", "gs_2048", 31, True),            # the same at full size (strips overlap at the last column)
 ])
 def test_forced_schedules_bit_exact(name, oracle, steps, resident):
