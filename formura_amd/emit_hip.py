@@ -956,7 +956,7 @@ def _stream3d_ok(em, ty, tz, cx, k):
         return False
     plane = (ty + 2 * k * s) * (tz + 2 * k * s)
     lds = sum(8 if ct == "double" else 4 for _, ct in em.fields) * k * (2 * s + 1) * plane
-    return lds <= 64 * 1024
+    return lds <= LDS_PER_WORKGROUP
 
 
 def _wave2d_ok(em, rows, k):
@@ -1024,11 +1024,14 @@ def candidate_specs(em) -> List[dict]:
             bpc = sum(8 if ct == "double" else 4 for _, ct in em.fields)
             s = em.cfg.sleeve
             for ty, tz, cx in ((8, 32, 16), (8, 32, 32), (4, 64, 16), (8, 16, 32), (16, 32, 16), (4, 32, 32),
-                               (4, 16, 32), (8, 8, 32)):
+                               (4, 16, 32), (8, 8, 32), (16, 32, 32), (8, 64, 32), (16, 16, 32), (32, 32, 32)):
                 if (em.N[1] % ty == 0 and em.N[2] % tz == 0 and em.N[0] % cx == 0
-                        and bpc * (2 * s + 1) * (ty + 2 * s) * (tz + 2 * s) <= 64 * 1024
+                        and bpc * (2 * s + 1) * (ty + 2 * s) * (tz + 2 * s) <= LDS_PER_WORKGROUP
                         and (em.N[0] // cx) * (em.N[1] // ty) * (em.N[2] // tz) >= 128):
                     out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": 1, "inline": 1})
+                    if ty * tz >= 512:
+                        out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": 1, "inline": 1,
+                                    "threads": 512})
         return out
     if em.dim == 2:
         for tx, ty in ((16, 32), (32, 32), (16, 64), (32, 64), (64, 32), (64, 64), (8, 128), (16, 128)):
