@@ -187,15 +187,17 @@ def run_heat_roofline(name: str = "heat3d_512", steps: int = 200):
 
 
 def run_euler(steps: int = 200):
-    """3-D Euler HLLC at 256^3: the plain FP64 path and the mixed form (FP32 flux
+    """3-D Euler HLLC at 256^3: the plain FP64 path and two mixed forms (FP32 flux
     arithmetic; FP64 wave speeds, positivity checks, update and totals) from the
-    same seeded state; error of the mixed run against FP64 per field."""
+    same seeded state -- fluxes recomputed in place (``euler3d_256_mixed``) and
+    fluxes evaluated once per face into LDS in FP32 (``euler3d_256_mixed_staged``);
+    error of each mixed run against FP64 per field."""
     import numpy as np
     from formura_amd import inputs
     from formura_amd.runtime import open_program
     st, out = None, {"steps": steps}
     res = {}
-    for name in ("euler3d_256", "euler3d_256_mixed"):
+    for name in ("euler3d_256", "euler3d_256_mixed", "euler3d_256_mixed_staged"):
         lib = open_program(name)
         lib.init()
         st = st or inputs.euler_state(1, lib.grid)
@@ -217,6 +219,10 @@ def run_euler(steps: int = 200):
     a, b = res["euler3d_256"], res["euler3d_256_mixed"]
     out["mixed_rel_maxnorm_err_vs_fp64"] = {f: float(np.max(np.abs(a[f] - b[f])) / np.max(np.abs(a[f]))) for f in a}
     out["mixed_speedup"] = out["euler3d_256_mixed"]["glups"] / out["euler3d_256"]["glups"]
+    c = res["euler3d_256_mixed_staged"]
+    out["mixed_staged_rel_maxnorm_err_vs_fp64"] = {f: float(np.max(np.abs(a[f] - c[f])) / np.max(np.abs(a[f])))
+                                                   for f in a}
+    out["mixed_staged_speedup"] = out["euler3d_256_mixed_staged"]["glups"] / out["euler3d_256"]["glups"]
     return out
 
 

@@ -55,6 +55,8 @@ PROGRAMS: Dict[str, tuple] = {
     "euler3d_16_inline": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1}),
     "euler3d_32_inline_stream": ("euler3d.fmr", "euler3d_32.yaml",
                                  {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 1, "inline": 1}),
+    "euler3d_16_staged": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "staged3d", "ty": 8, "tz": 16, "cx": 4}),
+    "euler3d_32_staged": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "staged3d", "ty": 16, "tz": 16, "cx": 8, "threads": 512}),
     "gs_64_tile_k8": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 8}),
     "gs_64_tile_k5_inplace": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 5, "inplace": 1}),
     "gs_64_wave_k3": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "wave2d", "rows": 16, "k": 3}),
@@ -64,8 +66,13 @@ PROGRAMS: Dict[str, tuple] = {
     # mixed precision: FP32 flux arithmetic, FP64 wave speeds, positivity checks and update
     "euler3d_16_mixed": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),
     "euler3d_32_mixed": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),
+    "euler3d_32_mixed_mat": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "flat", "mixed": 1}),
+    "euler3d_32_mixed_staged": ("euler3d.fmr", "euler3d_32.yaml",
+                                {"kind": "staged3d", "ty": 8, "tz": 16, "cx": 8, "mixed": 1}),
     "euler3d_32_mixed_stream": ("euler3d.fmr", "euler3d_32.yaml",
                                 {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 1, "inline": 1, "mixed": 1}),
+    "euler3d_256_mixed_staged": ("euler3d.fmr", "euler3d_256.yaml",
+                                 {"kind": "staged3d", "ty": 4, "tz": 64, "cx": 32, "threads": 512, "mixed": 1}),
     "euler3d_256_mixed": ("euler3d.fmr", "euler3d_256.yaml",
                           {"kind": "stream3d", "ty": 8, "tz": 32, "cx": 32, "k": 1, "inline": 1, "mixed": 1}),
 }

@@ -801,6 +801,290 @@ __global__ void __launch_bounds__(256) k_step_wave(const char* __restrict__ st_i
                 "    g_rt.launches += 1; steps -= k;\n  }\n}" % (self.k, self.k))
 
 
+def _merge_nodes(nodes, precs=None):
+    """One micro DAG evaluating several nodes at the same point: identical
+    instructions are shared (so a group of flux components computes its wave
+    speeds once); ``precs``: each node's per-micro precision (mixed form), part
+    of the sharing key.  Returns (node-like object, index of each node's value,
+    the merged per-micro precision or None)."""
+    from .manifest import Micro
+    merged, memo, roots = [], {}, []
+    mprec = {} if precs is not None else None
+    for ni, node in enumerate(nodes):
+        pr = precs[ni] if precs is not None else None
+        remap = {}
+        for j, m in enumerate(node.micro):
+            inst = m.inst
+            k = inst[0]
+            if k == "Uniop":
+                inst = (k, inst[1], remap[inst[2]])
+            elif k == "Binop":
+                inst = (k, inst[1], remap[inst[2]], remap[inst[3]])
+            elif k == "Triop":
+                inst = (k, inst[1], remap[inst[2]], remap[inst[3]], remap[inst[4]])
+            elif k not in ("Imm", "LoadCursorStatic"):
+                raise EmitError("staged3d: temporaries may read only the state (found %s)" % k)
+            p = pr.get(j) if pr else None
+            key = (inst, m.typ, m.src if k == "Imm" else None, p)
+            if key not in memo:
+                memo[key] = len(merged)
+                if p is not None:
+                    mprec[len(merged)] = p
+                merged.append(Micro(inst, m.typ, src=m.src))
+            remap[j] = memo[key]
+        roots.append(remap[len(node.micro) - 1])
+
+    class _Node:
+        pass
+    out = _Node()
+    out.micro, out.omid, out.name = merged, None, "group"
+    return out, roots, mprec
+
+
+class Staged3DSchedule:
+    """3-D, one level of manifest temporaries read only the state (flux
+    components of a finite-volume update): stream along x, one step per
+    launch, every temporary evaluated once per point into LDS.
+
+    A workgroup owns a TY x TZ column tile and a chunk of CX planes.  The
+    temporaries are grouped by the set of offsets at which the update reads
+    them (an x-flux at faces x-1 and x, a y-flux at faces y-1 and y, ...).  At
+    output plane x the workgroup loads the newest state plane into a ring of
+    DS planes (with the y/z halo every read needs), evaluates each group's
+    newest plane over the group's region (the tile plus the group's y/z read
+    offsets) into a ring of DG planes, one merged instruction DAG per point,
+    and then evaluates the update of plane x from the rings.  A face flux is
+    computed once instead of once per neighbouring cell.  Ring depths, halos
+    and the warm-up planes before the chunk follow from the offsets, so every
+    value is the one the flat schedule materializes: bit-identical."""
+
+    name = "staged3d"
+
+    def __init__(self, ty, tz, cx, threads=256):
+        self.ty, self.tz, self.cx = ty, tz, cx
+        self.k = 1
+        self.threads = int(threads)
+
+    @staticmethod
+    def analyse(em, ty, tz):
+        plan = em.step
+        mm, s = plan.mm, plan.sleeve
+        if em.dim != 3 or not plan.temps:
+            raise EmitError("staged3d needs a 3-D program with manifest temporaries")
+        O = {t: set() for t in plan.temps}
+        store_state = set()
+        for k in plan.stores:
+            base = plan.rng[k].offset - 2 * s
+            for m in mm[k].micro:
+                if m.inst[0] == "LoadCursor":
+                    O[m.inst[2]].add(tuple(c + base for c in m.inst[1]))
+                elif m.inst[0] == "LoadCursorStatic":
+                    store_state.add(tuple(c + base for c in m.inst[1]))
+                elif m.inst[0] == "LoadIndex":
+                    raise EmitError("staged3d: index reads are not supported")
+        groups = {}
+        for t in plan.temps:
+            for m in mm[t].micro:
+                if m.inst[0] in ("LoadCursor", "LoadIndex"):
+                    raise EmitError("staged3d: temporaries may read only the state")
+            if not O[t]:
+                continue
+            groups.setdefault(frozenset(O[t]), []).append(t)
+        G = []
+        for key, ts in sorted(groups.items(), key=lambda kv: min(kv[1])):
+            offs = list(key)
+            D = sorted(set(tuple(m.inst[1]) for t in ts for m in mm[t].micro if m.inst[0] == "LoadCursorStatic"))
+            g = {"temps": ts, "oxmin": min(o[0] for o in offs), "oxmax": max(o[0] for o in offs),
+                 "ymin": min(o[1] for o in offs), "ymax": max(o[1] for o in offs),
+                 "zmin": min(o[2] for o in offs), "zmax": max(o[2] for o in offs), "D": D}
+            g["DG"] = g["oxmax"] - g["oxmin"] + 1
+            g["RY"] = ty + g["ymax"] - g["ymin"]
+            g["RZ"] = tz + g["zmax"] - g["zmin"]
+            G.append(g)
+        planes, ys, zs = set(), [], []
+        for g in G:
+            for d in g["D"]:
+                planes.add(g["oxmax"] + d[0])
+                ys += [g["ymin"] + d[1], ty - 1 + g["ymax"] + d[1]]
+                zs += [g["zmin"] + d[2], tz - 1 + g["zmax"] + d[2]]
+        for o in store_state:
+            planes.add(o[0])
+            ys += [o[1], ty - 1 + o[1]]
+            zs += [o[2], tz - 1 + o[2]]
+        st = {"Smin": min(planes), "Smax": max(planes), "SYmin": min(ys), "SYmax": max(ys),
+              "SZmin": min(zs), "SZmax": max(zs)}
+        st["DS"] = st["Smax"] - st["Smin"] + 1
+        st["SRY"] = st["SYmax"] - st["SYmin"] + 1
+        st["SRZ"] = st["SZmax"] - st["SZmin"] + 1
+        st["W"] = max(g["DG"] for g in G) - 1 + st["DS"] - 1
+        return G, st
+
+    @staticmethod
+    def lds_bytes(em, ty, tz):
+        G, st = Staged3DSchedule.analyse(em, ty, tz)
+        bpv = {n: (8 if ct == "double" else 4) for n, ct in em.fields}
+        state = sum(bpv.values()) * st["DS"] * st["SRY"] * st["SRZ"]
+        tmp = sum(g["DG"] * g["RY"] * g["RZ"] * (8 if em.step.temp_ctype(t) == "double" else 4)
+                  for g in G for t in g["temps"])
+        return state + tmp
+
+    def kernels(self, em, plan):
+        ty, tz, cx, nth = self.ty, self.tz, self.cx, self.threads
+        G, st = self.analyse(em, ty, tz)
+        mm = plan.mm
+        fields = em.fields
+        nf = len(fields)
+        tctype = {t: plan.temp_ctype(t) for t in plan.temps}
+        L = []
+        L += ["  __shared__ %s S%d[%d][%d];" % (ct, fi, st["DS"], st["SRY"] * st["SRZ"])
+              for fi, (n, ct) in enumerate(fields)]
+        for g in G:
+            L += ["  __shared__ %s T%d[%d][%d];" % (tctype[t], t, g["DG"], g["RY"] * g["RZ"]) for t in g["temps"]]
+        lds = "\n".join(L)
+        # group evaluation: one loop over the concatenated regions of every group,
+        # so that all work-items are busy whatever the size of each region
+        gbodies = ["{"]
+        starts, tot = [], 0
+        for g in G:
+            starts.append(tot)
+            tot += g["RY"] * g["RZ"]
+        gbodies.append("  for (int gidx = threadIdx.x; gidx < %d; gidx += %d) {" % (tot, nth))
+        for gi, g in enumerate(G):
+            precs = [em.prec_of(mm[t]) or {} for t in g["temps"]] if em.mixed else None
+            node, roots, mprec = _merge_nodes([mm[t] for t in g["temps"]], precs)
+            dxs = sorted(set(d[0] for d in g["D"]))
+
+            def load(field, c, gi=gi):
+                fi = em.field_index[field]
+                return "S%d[sl_%d_%s][so + (%d)*%d + (%d)]" % (fi, gi, str(c[0]).replace("-", "m"), c[1], st["SRZ"], c[2])
+
+            b = Body(node, load, lambda *a: None, lambda ax: None, lambda n, v: "", "g%d_" % gi, prec=mprec)
+            kw = "if" if gi == 0 else "} else if"
+            # planes of this group computed before iteration x0 - (DG - 1) are never read
+            lines = ["%s (gidx < %d) {  // group %d: temporaries %s, read at x offsets %d..%d"
+                     % (kw, starts[gi] + g["RY"] * g["RZ"], gi, g["temps"], g["oxmin"], g["oxmax"]),
+                     "  if (x >= x0 - %d) {" % (g["DG"] - 1),
+                     "    const int P = x + (%d);" % g["oxmax"],
+                     "    const int gsl = (P - xs + 64 * %d) %% %d;" % (g["DG"], g["DG"])]
+            lines += ["    const int sl_%d_%s = (P + (%d) - xs + 64 * DS) %% DS;" % (gi, str(d).replace("-", "m"), d) for d in dxs]
+            lines.append("    const int idx = gidx - %d;" % starts[gi])
+            lines.append("    const int r = idx / %d, c = idx - r * %d;" % (g["RZ"], g["RZ"]))
+            lines.append("    const int so = (r + (%d)) * SRZ + (c + (%d));" % (g["ymin"] - st["SYmin"], g["zmin"] - st["SZmin"]))
+            lines += ["    " + l for l in b.lines]
+            lines += ["    T%d[gsl][idx] = (%s)g%d_%d;" % (t, tctype[t], gi, roots[i]) for i, t in enumerate(g["temps"])]
+            lines.append("  }")
+            gbodies += ["    " + l for l in lines]
+        gbodies += ["    }", "  }", "}"]
+        # the update
+        gof = {t: (gi, g) for gi, g in enumerate(G) for t in g["temps"]}
+        sbody = []
+        xslots = set()
+        for k in plan.stores:
+            node = mm[k]
+            base = plan.rng[k].offset - 2 * plan.sleeve
+
+            def sload(field, c, base=base):
+                fi = em.field_index[field]
+                o = [c[a] + base for a in range(3)]
+                xslots.add(("s", o[0]))
+                return "S%d[ss_%s][(r + (%d)) * SRZ + (c + (%d))]" % (fi, str(o[0]).replace("-", "m"),
+                                                                      o[1] - st["SYmin"], o[2] - st["SZmin"])
+
+            def tload(omid, c, base=base):
+                gi, g = gof[omid]
+                o = [c[a] + base for a in range(3)]
+                xslots.add(("t%d" % gi, o[0]))
+                return "T%d[ts_%d_%s][(r + (%d)) * %d + (c + (%d))]" % (omid, gi, str(o[0]).replace("-", "m"),
+                                                                     o[1] - g["ymin"], g["RZ"], o[2] - g["zmin"])
+
+            def store(fname, v):
+                return "FLD%d(st_out)[gout] = %s;" % (em.field_index[fname], v)
+
+            b = Body(node, sload, tload, lambda ax: None, store, "u%d_" % k, prec=em.prec_of(node))
+            sbody += ["{"] + ["  " + l for l in b.lines] + ["}"]
+        slot_lines = []
+        for kind, o in sorted(xslots):
+            if kind == "s":
+                slot_lines.append("const int ss_%s = (x + (%d) - xs + 64 * DS) %% DS;" % (str(o).replace("-", "m"), o))
+            else:
+                gi = int(kind[1:])
+                dg = G[gi]["DG"]
+                slot_lines.append("const int ts_%d_%s = (x + (%d) - xs + 64 * %d) %% %d;"
+                                  % (gi, str(o).replace("-", "m"), o, dg, dg))
+        reach = 1 + max(st["W"] - st["Smin"], abs(st["Smax"]), -st["SYmin"], st["SYmax"], -st["SZmin"], st["SZmax"])
+        wrap = [em.wrap_fn(a, reach) for a in range(3)]
+        return ["""
+__global__ void __launch_bounds__(%(NTH)d) k_step_staged(const char* __restrict__ st_in, char* __restrict__ st_out) {
+  constexpr int TY = %(TY)d, TZ = %(TZ)d, CX = %(CX)d;
+  constexpr int NTY = N2 / TY, NTZ = N3 / TZ, NT = NTY * NTZ * (N1 / CX);
+  constexpr int DS = %(DS)d, SRY = %(SRY)d, SRZ = %(SRZ)d;
+%(lds)s
+  const int b = blockIdx.x;
+  // consecutive workgroups go to different XCDs: neighbouring tiles share an L2
+  const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
+  const int zt = tile %% NTZ, yt = (tile / NTZ) %% NTY, xc = tile / (NTZ * NTY);
+  const int x0 = xc * CX, y0 = yt * TY, z0 = zt * TZ;
+  const int xs = x0 - %(W)d;  // warm-up: the rings fill before the first output plane
+  for (int x = xs; x < x0 + CX; ++x) {
+    __syncthreads();  // every read of the ring slots reused below is done
+    {  // state plane x + Smax into its slot
+      const int P = x + (%(SMAX)d);
+      const int sl = (P - xs + 64 * DS) %% DS;
+      const long long gp = (long long)%(w0)s(P) * N2;
+      for (int idx = threadIdx.x; idx < SRY * SRZ; idx += %(NTH)d) {
+        const int r = idx / SRZ, c = idx - r * SRZ;
+        const long long g = (gp + %(w1)s(y0 + (%(SYMIN)d) + r)) * N3 + %(w2)s(z0 + (%(SZMIN)d) + c);
+%(sload)s
+      }
+    }
+    __syncthreads();
+%(groups)s
+    __syncthreads();
+    if (x >= x0) {
+%(slots)s
+      for (int idx = threadIdx.x; idx < TY * TZ; idx += %(NTH)d) {
+        const int r = idx / TZ, c = idx - r * TZ;
+        const long long gout = ((long long)x * N2 + (y0 + r)) * N3 + (z0 + c);
+%(sbody)s
+      }
+    }
+  }
+}
+""" % dict(NTH=nth, TY=ty, TZ=tz, CX=cx, DS=st["DS"], SRY=st["SRY"], SRZ=st["SRZ"], lds=lds, W=st["W"],
+           SMAX=st["Smax"], SYMIN=st["SYmin"], SZMIN=st["SZmin"], w0=wrap[0], w1=wrap[1], w2=wrap[2],
+           sload="\n".join("        S%d[sl][idx] = FLD%d(st_in)[g];" % (fi, fi) for fi in range(nf)),
+           groups="\n".join("    " + l for l in gbodies),
+           slots="\n".join("      " + l for l in slot_lines),
+           sbody="\n".join("        " + l for l in sbody))]
+
+    def launch(self, em, plan):
+        nb = (em.N[0] // self.cx) * (em.N[1] // self.ty) * (em.N[2] // self.tz)
+        return ("static void run_step_staged(const char* in, char* out, hipStream_t s) {\n"
+                "  hipLaunchKernelGGL(k_step_staged, dim3(%d), dim3(%d), 0, s, in, out);\n"
+                "  FMR_CHECK(hipGetLastError());\n}\n" % (nb, self.threads))
+
+    def advance(self, em):
+        return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
+                "  (void)n0;\n"
+                "  for (long long t = 0; t < steps; ++t) {\n"
+                "    run_step_staged(g_rt.cur, g_rt.nxt, g_rt.stream); g_rt.swap();\n"
+                "    g_rt.launches += 1;\n  }\n}")
+
+
+def _staged3d_ok(em, ty, tz, cx, threads=256):
+    plan = em.step
+    if em.dim != 3 or not plan.temps or em.inlined:
+        return False
+    if em.N[0] % cx or em.N[1] % ty or em.N[2] % tz:
+        return False
+    try:
+        G, st = Staged3DSchedule.analyse(em, ty, tz)
+        lds = Staged3DSchedule.lds_bytes(em, ty, tz)
+    except EmitError:
+        return False
+    return lds <= LDS_PER_WORKGROUP and st["W"] < min(em.N) and cx >= 1
+
+
 class Stream3DSchedule:
     """3-D, no temporaries: stream along the slowest axis with K levels in LDS.
 
@@ -1005,6 +1289,11 @@ def schedule_from_spec(em, spec: dict):
         if not _wave2d_ok(em, *args):
             raise EmitError("wave2d %r does not apply to %s" % (spec, em.name))
         return Wave2DSchedule(*args, unroll=int(spec.get("unroll", 1)))
+    if kind == "staged3d":
+        args = (int(spec["ty"]), int(spec["tz"]), int(spec["cx"]))
+        if not _staged3d_ok(em, *args):
+            raise EmitError("staged3d %r does not apply to %s" % (spec, em.name))
+        return Staged3DSchedule(*args, threads=int(spec.get("threads", 256)))
     if kind == "stream3d":
         args = (int(spec["ty"]), int(spec["tz"]), int(spec["cx"]), int(spec["k"]))
         if not _stream3d_ok(em, *args):
@@ -1032,6 +1321,17 @@ def candidate_specs(em) -> List[dict]:
                     if ty * tz >= 512:
                         out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": 1, "inline": 1,
                                     "threads": 512})
+            # ... or evaluating each of them once per point into LDS rings
+            for ty, tz in ((8, 16), (16, 16), (8, 32), (16, 32), (4, 64), (8, 64)):
+                for cx in (8, 16, 32):
+                    if (_staged3d_ok(em, ty, tz, cx)
+                            and (em.N[0] // cx) * (em.N[1] // ty) * (em.N[2] // tz) >= 256):
+                        for nth in (256, 512):
+                            if nth <= 2 * ty * tz:
+                                spec = {"kind": "staged3d", "ty": ty, "tz": tz, "cx": cx}
+                                if nth != 256:
+                                    spec["threads"] = nth
+                                out.append(spec)
         return out
     if em.dim == 2:
         for tx, ty in ((16, 32), (32, 32), (16, 64), (32, 64), (64, 32), (64, 64), (8, 128), (16, 128)):

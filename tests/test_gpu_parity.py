@@ -145,6 +145,8 @@ def _euler_seed(lib):
     ("euler3d_32", "euler3d_32", 5, True),
     ("euler3d_16_inline", "euler3d_16", 8, False),           # temporaries recomputed in one kernel
     ("euler3d_32_inline_stream", "euler3d_32", 6, True),     # ... and streamed through LDS
+    ("euler3d_16_staged", "euler3d_16", 7, False),           # each face flux once, staged in LDS rings
+    ("euler3d_32_staged", "euler3d_32", 6, True),            # ... 16x16 tiles, 512 work-items
 ])
 def test_euler_hllc_bit_exact(name, oracle, steps, resident):
     o, h = _run_pair(name, oracle, _euler_seed, steps, resident=resident)
