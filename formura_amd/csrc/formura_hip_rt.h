@@ -62,6 +62,21 @@ __host__ __device__ __forceinline__ double fmr_divc(double a, double b, double y
   return fma(-r, y, q);
 }
 
+// The same with one correction, for divisors whose host-rounded reciprocal is
+// close enough: if |b*y - 1| <= 2^-54 then q0 = RN(a*y) is already faithful
+// (|a*y - a/b| <= |a/b| 2^-54 < ulp(a/b)/2, so q0 is one of the two doubles
+// around a/b, also where a*y and a/b lie in different binades), and one
+// Markstein step q1 = q0 - (b*q0 - a)*y gives RN(a/b).  The generator checks
+// the condition in exact rational arithmetic (b = RN(1e-4) with y = 1e4:
+// |b*y - 1| = 0.043 * 2^-53).  Range handling as in fmr_divc.
+__host__ __device__ __forceinline__ double fmr_divc1(double a, double b, double y) {
+  const double aa = fabs(a);
+  if (__builtin_expect(!((aa >= 0x1p-900 && aa <= 0x1p+900) || aa == 0.0), 0)) return a / b;
+  const double q = a * y;
+  const double r = fma(b, q, -a);
+  return fma(-r, y, q);
+}
+
 // Positivity checks of the mixed-precision form: every FP64 square-root
 // argument (e.g. gamma*p/rho of a wave speed) must be > 0; a non-positive or
 // NaN argument is counted (vector atomic, taken only on a violation).

@@ -171,6 +171,12 @@ def _fold(m, vals, typ):
     return None
 
 
+def _divc_one_step(b: float) -> bool:
+    """``fmr_divc1`` applies: |b*RN(1/b) - 1| <= 2^-54, exactly (b > 0)."""
+    y = 1.0 / b
+    return abs(Fraction(b) * Fraction(y) - 1) <= Fraction(1, 2 ** 54)
+
+
 def _divc_ok(b) -> bool:
     """A constant divisor that ``fmr_divc`` handles: finite, 2^-100 <= |b| <= 2^100."""
     if isinstance(b, bool):
@@ -208,7 +214,8 @@ class Body:
         (device code: the correctly rounded quotient from the host-rounded
         reciprocal and two fused corrections) or, for a power-of-two
         divisor, as the exact product with its reciprocal; host code keeps
-        the division."""
+        the division (with one correction instead of two where the divisor
+        allows it: ``fmr_divc1``)."""
         self.lines: List[str] = []
         vals: Dict[int, object] = {}
         params = params or {}
@@ -260,10 +267,12 @@ class Body:
                 if mant == 0.5:
                     # 1/b is exact: a*(1/b) and a/b are the same real number, rounded once
                     rhs = "%s*%s" % (src, show_double(1.0 / b))
-                elif b > 0:
-                    rhs = "fmr_divc(%s, %s, %s)" % (src, show_double(b), show_double(1.0 / b))
                 else:
-                    rhs = "-fmr_divc(%s, %s, %s)" % (src, show_double(-b), show_double(-1.0 / b))
+                    fn = "fmr_divc1" if _divc_one_step(abs(b)) else "fmr_divc"
+                    if b > 0:
+                        rhs = "%s(%s, %s, %s)" % (fn, src, show_double(b), show_double(1.0 / b))
+                    else:
+                        rhs = "-%s(%s, %s, %s)" % (fn, src, show_double(-b), show_double(-1.0 / b))
                 self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, rhs))
                 continue
             if fold:

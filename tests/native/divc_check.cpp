@@ -1,5 +1,7 @@
-// Host check of fmr_divc (formura_hip_rt.h): the division of a by a constant b
-// through y = RN(1/b) and two fused corrections must equal IEEE a/b bit for bit.
+// Host check of fmr_divc and fmr_divc1 (formura_hip_rt.h): the division of a by
+// a constant b through y = RN(1/b) and two fused corrections (one, where
+// |b*y - 1| <= 2^-54) must equal IEEE a/b bit for bit.  b*y - 1 is exact as a
+// fused multiply-add: it is below 2^-52 in magnitude and needs at most 53 bits.
 // Cases: random dividends over the fast range, dividends whose quotient lies
 // next to a rounding midpoint (the hard cases of correct rounding), stencil-like
 // differences, and specials (signed zeros, subnormals, the range limits, Inf, NaN).
@@ -14,11 +16,18 @@ static inline double dbl(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
 int main(int argc, char** argv) {
   long n = argc > 1 ? atol(argv[1]) : 10000000;
   if (argc > 2) s = strtoull(argv[2], nullptr, 0);
-  double bs[] = {1e-4, 0.01, 3.0, 7.0, 0.1, 1.0/3.0, 20.48, 1e-5, 1.7976931348623157e+30, 0x1.fffffffffffffp-1, 0x1.0000000000001p0, 9.999999999999999e-5};
-  long bad = 0, tot = 0;
-  for (double b : bs) {
+  // fixed divisors, then random ones over [2^-100, 2^100) with fewer dividends each
+  const int NFIX = 12, NRND = 48;
+  double bs[NFIX + NRND] = {1e-4, 0.01, 3.0, 7.0, 0.1, 1.0/3.0, 20.48, 1e-5, 1.7976931348623157e+30, 0x1.fffffffffffffp-1, 0x1.0000000000001p0, 9.999999999999999e-5};
+  for (int k = 0; k < NRND; ++k) bs[NFIX + k] = ldexp(1.0 + (double)(nx() >> 12) * 0x1p-52, (int)(nx() % 200) - 100);
+  long bad = 0, tot = 0, one_step = 0;
+  for (int bi = 0; bi < NFIX + NRND; ++bi) {
+    const double b = bs[bi];
     double y = 1.0 / b;
-    for (long i = 0; i < n; ++i) {
+    const bool one = fabs(fma(b, y, -1.0)) <= 0x1p-54;
+    one_step += one;
+    const long nb = bi < NFIX ? n : n / 16;
+    for (long i = 0; i < nb; ++i) {
       uint64_t r = nx();
       double a;
       int kind = i % 3;
@@ -38,10 +47,21 @@ int main(int argc, char** argv) {
       double q1 = fmr_divc(a, b, y), q2 = a / b;
       ++tot;
       if (bits(q1) != bits(q2)) { if (bad < 10) printf("MISMATCH b=%a a=%a got %a want %a\n", b, a, q1, q2); ++bad; }
+      if (one) {
+        const double q3 = fmr_divc1(a, b, y);
+        ++tot;
+        if (bits(q3) != bits(q2)) { if (bad < 10) printf("MISMATCH (one step) b=%a a=%a got %a want %a\n", b, a, q3, q2); ++bad; }
+      }
     }
     double sp[] = {0.0, -0.0, 1e-310, -1e-310, 1e308, -1e308, INFINITY, -INFINITY, NAN, 0x1p-900, 0x1p900, 0x1p-901, 0x1.fffffffffffffp900, 5e-324};
-    for (double a : sp) { double q1 = fmr_divc(a, b, y), q2 = a / b; ++tot; if (bits(q1) != bits(q2) && !(q1 != q1 && q2 != q2)) { printf("SPECIAL MISMATCH b=%a a=%a got %a want %a\n", b, a, q1, q2); ++bad; } }
+    for (double a : sp) {
+      double q1 = fmr_divc(a, b, y), q2 = a / b, q3 = one ? fmr_divc1(a, b, y) : q2;
+      ++tot;
+      if (bits(q1) != bits(q2) && !(q1 != q1 && q2 != q2)) { printf("SPECIAL MISMATCH b=%a a=%a got %a want %a\n", b, a, q1, q2); ++bad; }
+      if (bits(q3) != bits(q2) && !(q3 != q3 && q2 != q2)) { printf("SPECIAL MISMATCH (one step) b=%a a=%a got %a want %a\n", b, a, q3, q2); ++bad; }
+    }
   }
+  printf("divisors %d, one-step divisors %ld\n", NFIX + NRND, one_step);
   printf("checked %ld, mismatches %ld\n", tot, bad);
   return bad != 0;
 }

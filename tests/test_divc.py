@@ -7,6 +7,13 @@ divides.  The same header is compiled for the host (x86 FMA is correctly
 rounded, as the device's is) and checked on random, near-midpoint and special
 dividends for several divisors.  On the GPU the parity tests then compare the
 kernels themselves with the oracle.
+
+``fmr_divc1`` (one correction) is emitted only for divisors with
+|b*RN(1/b) - 1| <= 2^-54, the condition under which q0 = RN(a*y) is faithful
+and Markstein's theorem applies; the checker tests it on those divisors.  A
+failure of one correction elsewhere needs a quotient within about 2^-52 ulp of
+a rounding midpoint, which random and near-midpoint dividends do not reach: the
+condition rests on the proof in the header, not on this test.
 """
 import os
 import subprocess
@@ -42,10 +49,17 @@ def test_emitted_divisions():
     from formura_amd.emit_hip import generate
     _, src, _ = generate(os.path.join(build.PROG_DIR, "grayscott2d.fmr"),
                          os.path.join(build.PROG_DIR, "gs_64.yaml"), "gs_64")
-    assert src.count(", 0.0001, 10000.0)") == 4  # d_xx and d_yy of u and of v
+    assert src.count("fmr_divc1(") == 4 and src.count(", 0.0001, 10000.0)") == 4  # d_xx and d_yy of u and of v
     setup = src[src.index("static void Formura_Setup"):]
     assert "fmr_divc" not in setup[:setup.index("\n}\n")]
     _, hsrc, _ = generate(os.path.join(build.PROG_DIR, "heat3d.fmr"),
                           os.path.join(build.PROG_DIR, "heat3d_16.yaml"), "heat3d_16")
     kernels = hsrc[:hsrc.index("static void Formura_Setup")]
     assert "fmr_divc" not in kernels and "*256.0" in kernels
+
+
+def test_one_step_condition():
+    """The exact condition for one correction, on divisors either side of it."""
+    from formura_amd.emit import _divc_one_step
+    assert _divc_one_step(1e-4) and _divc_one_step(0.01) and _divc_one_step(3.0)
+    assert not _divc_one_step(1e-5) and not _divc_one_step(9.999999999999999e-5)
