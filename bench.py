@@ -83,11 +83,13 @@ def run_gpu(name: str, steps: int, warmup: int, bytes_per_lup: float, seed: int 
 
 
 def run_ensemble(count: int, steps: int):
-    """The batched FP32 (Kahan) Gray-Scott ensemble and its FP64 twin on the GPU.
+    """The batched FP32 Gray-Scott ensemble in its two forms (f32k: Kahan-compensated
+    accumulation; f32c: and the increment in float-float FP32) and its FP64 twin on
+    the GPU.
 
-    Returns GLUPS of both (all instances, whole ensemble, state resident) and the
-    worst relative L2 error of any instance/field of the FP32 run against its own
-    FP64 run."""
+    Returns GLUPS of each (all instances, whole ensemble, state resident) and the
+    relative L2 error of every instance of each FP32 run against its own FP64 run
+    (maximum, median, instances within 1e-5)."""
     import numpy as np
     from formura_amd import batch as B
     sizes, params, seeds = B.grayscott_ensemble(count)
@@ -96,7 +98,7 @@ def run_ensemble(count: int, steps: int):
     out = {"instances": count, "cells": cells, "steps": steps,
            "sides": [min(s[0] for s in sizes), max(s[0] for s in sizes)]}
     ens = {}
-    for name in ("gs_batch_f32k", "gs_batch_f64"):
+    for name in ("gs_batch_f32k", "gs_batch_f32c", "gs_batch_f64"):
         e = B.Ensemble(name, sizes, params)
         for i, st in enumerate(inits):
             e.set_state(i, st)
@@ -107,18 +109,22 @@ def run_ensemble(count: int, steps: int):
         out[name] = {"glups": cells * steps / (ms * 1e-3) / 1e9, "ms": ms,
                      "tile": e.desc["tile"], "steps_per_launch": e.desc["steps_per_launch"]}
         ens[name] = e
-    errs, rms = [], {"u": 0.0, "v": 0.0}
-    for i in range(count):
-        x, y = ens["gs_batch_f64"].get_state(i), ens["gs_batch_f32k"].get_state(i)
-        num = sum(float(np.sum((y[f] - x[f]) ** 2)) for f in ("u", "v"))
-        den = sum(float(np.sum(x[f] ** 2)) for f in ("u", "v"))
-        errs.append((num / den) ** 0.5)
-        for f in ("u", "v"):
-            rms[f] = max(rms[f], float(np.sqrt(np.mean((y[f] - x[f]) ** 2))))
-    errs = np.array(errs)
-    out["f32k_rel_l2_vs_f64"] = {"max": float(errs.max()), "median": float(np.median(errs)),
-                                 "instances_le_1e-5": int((errs <= 1e-5).sum())}
-    out["f32k_max_rms_error_per_field"] = rms
+    all_errs = {}
+    for prec in ("f32k", "f32c"):
+        errs, rms = [], {"u": 0.0, "v": 0.0}
+        for i in range(count):
+            x, y = ens["gs_batch_f64"].get_state(i), ens["gs_batch_" + prec].get_state(i)
+            num = sum(float(np.sum((y[f] - x[f]) ** 2)) for f in ("u", "v"))
+            den = sum(float(np.sum(x[f] ** 2)) for f in ("u", "v"))
+            errs.append((num / den) ** 0.5)
+            for f in ("u", "v"):
+                rms[f] = max(rms[f], float(np.sqrt(np.mean((y[f] - x[f]) ** 2))))
+        errs = np.array(errs)
+        all_errs[prec] = errs
+        out[prec + "_rel_l2_vs_f64"] = {"max": float(errs.max()), "median": float(np.median(errs)),
+                                        "instances_le_1e-5": int((errs <= 1e-5).sum())}
+        out[prec + "_max_rms_error_per_field"] = rms
+    errs = all_errs["f32k"]
     # Control: the FP64 ensemble again, from initial states perturbed at the scale
     # of FP32 rounding (u scaled by 1 + 2^-24 (2r - 1), r ~ U[0,1) seeded).  An
     # instance whose FP64 run moves by more than 1e-5 under that perturbation
@@ -140,7 +146,8 @@ def run_ensemble(count: int, steps: int):
     out["f64_perturbed_2^-24_rel_l2"] = {
         "median": float(np.median(sens)), "instances_le_1e-5": int((sens <= 1e-5).sum()),
         "f32k_above_1e-5_where_f64_control_le_1e-5": int(((errs > 1e-5) & (sens <= 1e-5)).sum()),
-        "f32k_le_1e-5_where_f64_control_above_1e-5": int(((errs <= 1e-5) & (sens > 1e-5)).sum())}
+        "f32k_le_1e-5_where_f64_control_above_1e-5": int(((errs <= 1e-5) & (sens > 1e-5)).sum()),
+        "f32c_above_1e-5_where_f64_control_le_1e-5": int(((all_errs["f32c"] > 1e-5) & (sens <= 1e-5)).sum())}
     for e in ens.values():
         e.close()
     return out
@@ -515,6 +522,7 @@ def main():
             pi = run_per_instance_fp64(sizes, args.ensemble_steps)
             ens["per_instance_fp64"] = pi
             ens["fp32_batched_vs_per_instance_fp64"] = ens["gs_batch_f32k"]["glups"] / pi["glups"]
+            ens["fp32c_batched_vs_per_instance_fp64"] = ens["gs_batch_f32c"]["glups"] / pi["glups"]
             ens["fp32_batched_vs_plain_fp64_2048"] = ens["gs_batch_f32k"]["glups"] / r["glups"]
             line["ensemble"] = ens
         except Exception as e:

@@ -144,6 +144,7 @@ def build_sample(name: str, force: bool = False) -> str:
 BATCHES: Dict[str, tuple] = {
     "gs_batch_f64": ("grayscott2d.fmr", "gs_64.yaml", "f64", ["f", "k"], 16, 32, 3),
     "gs_batch_f32k": ("grayscott2d.fmr", "gs_64.yaml", "f32k", ["f", "k"], 16, 32, 4),
+    "gs_batch_f32c": ("grayscott2d.fmr", "gs_64.yaml", "f32c", ["f", "k"], 16, 32, 4),
 }
 
 

@@ -16,6 +16,18 @@ Two precisions:
           compensation term c, updated as y = dq - c, t = q + y,
           c' = (t - q) - y, q' = t.  The compensation moves with the cell in the
           drifting frame and stays in LDS across the K fused steps.
+``f32c``  as ``f32k``, and the increment dq evaluated in float-float FP32
+          arithmetic (``ff_increment``): every value of dq is a pair (hi, lo)
+          of FP32 numbers built with error-free transformations -- TwoProduct
+          through the fused multiply-add, TwoSum (Knuth) -- and constants and
+          lifted parameters enter as their FP32 pairs; only the stencil parts
+          (linear combinations of loads at two or more cursors with constant
+          factors: the Laplacian and its multiples) stay plain FP32.  The pair's
+          low part is folded into the Kahan step (y = dq_hi - (c - dq_lo)).
+          Near a steady pattern the point-local terms of a reaction nearly
+          cancel (-u v^2 + F (1 - u) in Gray-Scott), so their plain FP32
+          rounding is the same bias at every step and accumulates; the pairs
+          remove it.
 
 Kernel: the ``tile2d`` scheme (tile + low-side halo of 2*K*s in LDS, K steps,
 ping-pong buffers) over a table of tiles of all instances; a fixed-size grid of
@@ -32,7 +44,169 @@ from . import syntax as S
 from .emit import Body, tile_load, EmitError, GraphPlan, state_fields
 from .manifest import calc_range
 
-PRECISIONS = {"f64": ("double", False), "f32k": ("float", True)}
+PRECISIONS = {"f64": ("double", False), "f32k": ("float", True), "f32c": ("float", True)}
+
+
+def _f32(x: float) -> float:
+    import numpy as np
+    return float(np.float32(x))
+
+
+def _pow2(x: float) -> bool:
+    import math
+    return x != 0.0 and math.isfinite(x) and math.frexp(abs(x))[0] == 0.5
+
+
+def _flit(x: float) -> str:
+    """An FP32 literal that parses to exactly the float x."""
+    if x != x or x in (float("inf"), float("-inf")):
+        raise EmitError("non-finite constant in the float-float increment")
+    return "%.9ef" % x
+
+
+def ff_increment(node, body: Body, root: int, pidx: Dict[str, int], pre: str, stencil_plain: bool = True):
+    """Float-float evaluation of the sub-DAG of ``node`` below micro ``root`` (the
+    increment dq of an update q + dq).  ``body`` is the node's plain FP32 Body
+    (its values ``<prefix><j>`` and folded constants ``body.vals``); ``pidx``
+    maps lifted literals to their index in the parameter array P (FP64).
+    Returns (lines, hi, lo) with C expressions of the pair's parts.
+
+    Stencil-linear values that read two or more cursors keep their plain FP32
+    value (lo = 0); all other values are pairs.  Operations on pairs:
+      x + y: s = xh + yh, e = TwoSum error + xl + yl   (Knuth's TwoSum)
+      x * y: p = xh * yh, e = fma(xh, yh, -p) + xh*yl + xl*yh   (TwoProduct)
+      x / c: x * (hi, lo) of 1/c (c a constant, 1/c rounded in FP64)
+    A pair is left unnormalized between operations (|lo| stays near ulp(hi));
+    the result is normalized once (Fast2Sum).  Operations outside + - * / and
+    unary minus (square roots, comparisons, selections) take the hi parts."""
+    micro = node.micro
+    # the plain FP32 value of micro j is named like the Body's: prefix + j
+    prefix = body.result.rsplit("_", 1)[0] + "_"
+    need = set()
+    stack = [root]
+    while stack:
+        j = stack.pop()
+        if j in need:
+            continue
+        need.add(j)
+        m = micro[j]
+        k = m.inst[0]
+        if k == "Uniop":
+            stack.append(m.inst[2])
+        elif k == "Binop":
+            stack += [m.inst[2], m.inst[3]]
+        elif k == "Triop":
+            stack += [m.inst[2], m.inst[3], m.inst[4]]
+    const, lin, curs = {}, {}, {}
+    for j in sorted(need):
+        m = micro[j]
+        k = m.inst[0]
+        if m.src in pidx and k not in ("LoadCursorStatic", "LoadCursor", "LoadIndex"):
+            const[j] = ("param", pidx[m.src])
+        elif j in body.vals and isinstance(body.vals[j], float):
+            const[j] = ("value", body.vals[j])
+        elif j in body.vals:
+            const[j] = ("value", float(body.vals[j]))
+        if j in const:
+            lin[j], curs[j] = True, set()
+        elif k == "LoadCursorStatic":
+            lin[j], curs[j] = True, {(m.inst[2], tuple(m.inst[1]))}
+        elif k == "Uniop" and m.inst[1] == "-":
+            lin[j], curs[j] = lin[m.inst[2]], set(curs[m.inst[2]])
+        elif k == "Binop" and m.inst[1] in ("+", "-"):
+            a, b = m.inst[2], m.inst[3]
+            lin[j], curs[j] = lin[a] and lin[b], curs[a] | curs[b]
+        elif k == "Binop" and m.inst[1] in ("*", "/"):
+            a, b = m.inst[2], m.inst[3]
+            lin[j] = (a in const and lin[b] and m.inst[1] == "*") or (b in const and lin[a])
+            curs[j] = curs[a] | curs[b]
+        else:
+            lin[j], curs[j] = False, set().union(*[curs.get(x, set()) for x in m.inst[2:] if isinstance(x, int)])
+    plain = {j for j in need if j not in const and lin[j] and len(curs[j]) >= 2} if stencil_plain else set()
+    lines = []
+    pair = {}
+
+    def cpair(c):
+        kind, v = const[c]
+        if kind == "param":
+            return "Ph[%d]" % v, "Pl[%d]" % v
+        h = _f32(v)
+        lo = _f32(v - h)
+        return _flit(h), (_flit(lo) if lo != 0.0 else None)
+
+    def get(x):
+        if x in const:
+            return cpair(x)
+        if x in plain or micro[x].inst[0] == "LoadCursorStatic":
+            return "%s%d" % (prefix, x), None
+        return pair[x]
+
+    for j in sorted(need):
+        if j in const or j in plain or micro[j].inst[0] == "LoadCursorStatic":
+            continue
+        m = micro[j]
+        k = m.inst[0]
+        h, l = "%sh%d" % (pre, j), "%sl%d" % (pre, j)
+        if k == "Uniop" and m.inst[1] == "-":
+            xh, xl = get(m.inst[2])
+            lines.append("const float %s = -%s;" % (h, xh))
+            pair[j] = (h, None) if xl is None else (h, "(-%s)" % xl)
+            if xl is not None:
+                lines.append("const float %s = -%s;" % (l, xl))
+                pair[j] = (h, l)
+            continue
+        if k == "Binop" and m.inst[1] in ("+", "-"):
+            (xh, xl), (yh, yl) = get(m.inst[2]), get(m.inst[3])
+            if m.inst[1] == "-":
+                yh = "(-%s)" % yh
+                yl = None if yl is None else "(-%s)" % yl
+            lines.append("const float %s = %s + %s;" % (h, xh, yh))
+            lines.append("const float %sb%d = %s - %s;" % (pre, j, h, xh))
+            err = "((%s - (%s - %sb%d)) + (%s - %sb%d))" % (xh, h, pre, j, yh, pre, j)
+            extra = "".join(" + %s" % t for t in (xl, yl) if t is not None)
+            lines.append("const float %s = %s%s;" % (l, err, extra))
+            pair[j] = (h, l)
+            continue
+        if k == "Binop" and m.inst[1] in ("*", "/"):
+            a, b = m.inst[2], m.inst[3]
+            if m.inst[1] == "/":
+                if b not in const or const[b][0] != "value":
+                    raise EmitError("float-float increment: division by a non-constant")
+                r = 1.0 / const[b][1]
+                yh, yl = _flit(_f32(r)), _f32(r - _f32(r))
+                yl = _flit(yl) if yl != 0.0 else None
+            else:
+                yh, yl = get(b)
+            xh, xl = get(a)
+            if a in const and b not in const:
+                (xh, xl), (yh, yl), a, b = (yh, yl), (xh, xl), b, a
+            if b in const and const[b][0] == "value" and yl is None and _pow2(const[b][1] if m.inst[1] == "*" else 1.0 / const[b][1]):
+                # a power of two: the product is exact
+                lines.append("const float %s = %s * %s;" % (h, xh, yh))
+                if xl is None:
+                    pair[j] = (h, None)
+                else:
+                    lines.append("const float %s = %s * %s;" % (l, xl, yh))
+                    pair[j] = (h, l)
+                continue
+            lines.append("const float %s = %s * %s;" % (h, xh, yh))
+            e = "fmaf(%s, %s, -%s)" % (xh, yh, h)
+            if yl is not None:
+                e = "fmaf(%s, %s, %s)" % (xh, yl, e)
+            if xl is not None:
+                e = "fmaf(%s, %s, %s)" % (xl, yh, e)
+            lines.append("const float %s = %s;" % (l, e))
+            pair[j] = (h, l)
+            continue
+        # anything else: on the hi parts, in FP32 (the Body's expression with hi operands)
+        lines.append("const float %s = %s%d;" % (h, prefix, j))
+        pair[j] = (h, None)
+    rh, rl = get(root)
+    if rl is None:
+        return lines, rh, "0.0f"
+    lines.append("const float %sH = %s + %s;" % (pre, rh, rl))
+    lines.append("const float %sL = %s - (%sH - %s);" % (pre, rl, pre, rh))
+    return lines, "%sH" % pre, "%sL" % pre
 
 
 class BatchEmitter:
@@ -45,6 +219,7 @@ class BatchEmitter:
             raise EmitError("unknown precision %r" % precision)
         self.real, self.kahan = PRECISIONS[precision]
         self.precision = precision
+        self.compensated = precision == "f32c"
         self.params = list(params)
         self.tx, self.ty, self.k = tx, ty, k
         self.G = graph_launches + (graph_launches % 2)
@@ -73,6 +248,11 @@ class BatchEmitter:
         real = self.real
         pmap = {p: "P[%d]" % i for i, p in enumerate(self.params)}
         lines = []
+        if self.compensated:
+            # FP32 pairs of the FP64 parameters: P = Ph + Pl to about 2^-48
+            lines += ["float Ph[NP], Pl[NP];",
+                      "#pragma unroll",
+                      "for (int q = 0; q < NP; ++q) { Ph[q] = (float)P[q]; Pl[q] = (float)(P[q] - (double)Ph[q]); }"]
         for k in self.plan.stores:
             node = self.plan.mm[k]
             base = self.plan.rng[k].offset - 2 * s
@@ -86,7 +266,12 @@ class BatchEmitter:
                         params=pmap, real=("float" if real == "float" else None))
             lines.append("{")
             lines += ["  " + l for l in body.lines]
-            if self.kahan:
+            if self.compensated:
+                q, dq = self._update_ids(node, fname, base)
+                fl, hi, lo = ff_increment(node, body, dq, {p: i for i, p in enumerate(self.params)}, "c%d_" % k)
+                lines += ["  " + l for l in fl]
+                lines.append("  old%d = s%d_%d; inc%d = %s; incl%d = %s;" % (fi, k, q, fi, hi, fi, lo))
+            elif self.kahan:
                 q, dq = self._update_ids(node, fname, base)
                 lines.append("  old%d = s%d_%d; inc%d = s%d_%d;" % (fi, k, q, fi, k, dq))
             else:
@@ -94,6 +279,8 @@ class BatchEmitter:
             lines.append("}")
         params = ", ".join("const %s* __restrict__ lds%d" % (real, i) for i in range(nf))
         outs = ", ".join("%s& old%d, %s& inc%d" % (real, i, real, i) for i in range(nf))
+        if self.compensated:
+            outs += "".join(", float& incl%d" % i for i in range(nf))
         cbase = "(%d)*RY + (%d)" % (self.plan.rng[self.plan.stores[0]].offset - 2 * s,
                                     self.plan.rng[self.plan.stores[0]].offset - 2 * s)
         lds = "\n".join("  __shared__ %s B%d[2][RX * RY];" % (real, i) for i in range(nf))
@@ -109,12 +296,17 @@ class BatchEmitter:
         srcs = ", ".join("B%d[sb]" % i for i in range(nf))
         decl = " ".join("%s old%d, inc%d;" % (real, i, i) for i in range(nf))
         refs = ", ".join("old%d, inc%d" % (i, i) for i in range(nf))
+        if self.compensated:
+            decl += " " + " ".join("float incl%d;" % i for i in range(nf))
+            refs += "".join(", incl%d" % i for i in range(nf))
         if self.kahan:
-            # Kahan: ky = dq - c, kt = q + ky, kc = (kt - q) - ky (names that shadow nothing)
+            # Kahan: ky = dq - c, kt = q + ky, kc = (kt - q) - ky (names that shadow nothing);
+            # f32c adds the increment's low part: ky = dq_hi - (c - dq_lo)
+            cterm = ("(C%(i)d[sb][o + CB] - incl%(i)d)" if self.compensated else "C%(i)d[sb][o + CB]")
             upd = "\n".join(
-                "        const %(r)s ky%(i)d = inc%(i)d - C%(i)d[sb][o + CB];\n"
+                ("        const %(r)s ky%(i)d = inc%(i)d - " + cterm + ";\n"
                 "        const %(r)s kt%(i)d = old%(i)d + ky%(i)d;\n"
-                "        const %(r)s kc%(i)d = (kt%(i)d - old%(i)d) - ky%(i)d;" % {"r": real, "i": i} for i in range(nf))
+                "        const %(r)s kc%(i)d = (kt%(i)d - old%(i)d) - ky%(i)d;") % {"r": real, "i": i} for i in range(nf))
             st_lds = " ".join("B%d[db][o] = kt%d; C%d[db][o] = kc%d;" % (i, i, i, i) for i in range(nf))
             st_glb = " ".join("dst[%dLL * plane + g] = kt%d; dst[%dLL * plane + g] = kc%d;" % (i, i, nf + i, i)
                               for i in range(nf))

@@ -18,7 +18,7 @@ extern "C" {
 
 typedef struct FormuraBatch FormuraBatch;
 
-/* JSON: program, precision ("f64" or "f32k"), fields, parameter names, tile,
+/* JSON: program, precision ("f64", "f32k" or "f32c"), fields, parameter names, tile,
  * steps per launch. */
 const char *Formura_Batch_Describe(void);
 /* ninst instances; instance i has grid n1[i] x n2[i] (multiples of the tile)

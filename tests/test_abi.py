@@ -44,7 +44,7 @@ def test_dropin_driver_compiles_against_generated_header():
 
 
 @pytest.mark.parametrize("kind,names", [
-    ("batch", ["gs_batch_f64", "gs_batch_f32k"]),
+    ("batch", ["gs_batch_f64", "gs_batch_f32k", "gs_batch_f32c"]),
     ("adjoint", ["gs_64_adj", "gs_64_adj_f32", "heat3d_16_adj"]),
 ])
 def test_form_libraries_export_declared_symbols(kind, names):
