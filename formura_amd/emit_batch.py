@@ -22,7 +22,8 @@ Two precisions:
           through the fused multiply-add, TwoSum (Knuth) -- and constants and
           lifted parameters enter as their FP32 pairs; only the stencil parts
           (linear combinations of loads at two or more cursors with constant
-          factors: the Laplacian and its multiples) stay plain FP32.  The pair's
+          factors that are FP32 numbers: the Laplacian) stay plain FP32, and a
+          rounded coefficient (a diffusivity) multiplies them as a pair.  The pair's
           low part is folded into the Kahan step (y = dq_hi - (c - dq_lo)).
           Near a steady pattern the point-local terms of a reaction nearly
           cancel (-u v^2 + F (1 - u) in Gray-Scott), so their plain FP32
@@ -52,6 +53,15 @@ def _f32(x: float) -> float:
     return float(np.float32(x))
 
 
+def _f32_exact(c, reciprocal: bool) -> bool:
+    """Whether a folded constant (or its reciprocal) is an FP32 number to 2^-40."""
+    kind, v = c
+    if kind != "value" or v == 0.0:
+        return False
+    x = 1.0 / v if reciprocal else v
+    return abs(_f32(x) - x) <= 2.0 ** -40 * abs(x)
+
+
 def _pow2(x: float) -> bool:
     import math
     return x != 0.0 and math.isfinite(x) and math.frexp(abs(x))[0] == 0.5
@@ -71,7 +81,8 @@ def ff_increment(node, body: Body, root: int, pidx: Dict[str, int], pre: str, st
     maps lifted literals to their index in the parameter array P (FP64).
     Returns (lines, hi, lo) with C expressions of the pair's parts.
 
-    Stencil-linear values that read two or more cursors keep their plain FP32
+    Stencil-linear values that read two or more cursors, through + - and
+    constant factors that are FP32 numbers (to 2^-40), keep their plain FP32
     value (lo = 0); all other values are pairs.  Operations on pairs:
       x + y: s = xh + yh, e = TwoSum error + xl + yl   (Knuth's TwoSum)
       x * y: p = xh * yh, e = fma(xh, yh, -p) + xh*yl + xl*yh   (TwoProduct)
@@ -117,8 +128,13 @@ def ff_increment(node, body: Body, root: int, pidx: Dict[str, int], pre: str, st
             a, b = m.inst[2], m.inst[3]
             lin[j], curs[j] = lin[a] and lin[b], curs[a] | curs[b]
         elif k == "Binop" and m.inst[1] in ("*", "/"):
+            # a constant factor keeps the product in the plain stencil part only if
+            # its FP32 value is exact (to 2^-40): a rounded diffusion coefficient is
+            # the same relative bias at every step
             a, b = m.inst[2], m.inst[3]
-            lin[j] = (a in const and lin[b] and m.inst[1] == "*") or (b in const and lin[a])
+            div = m.inst[1] == "/"
+            lin[j] = ((a in const and lin[b] and not div and _f32_exact(const[a], False))
+                      or (b in const and lin[a] and _f32_exact(const[b], div)))
             curs[j] = curs[a] | curs[b]
         else:
             lin[j], curs[j] = False, set().union(*[curs.get(x, set()) for x in m.inst[2:] if isinstance(x, int)])

@@ -2,7 +2,8 @@
 
 ``emit_batch.ff_increment`` turns the increment dq of each update q + dq into
 FP32 pairs (hi, lo) built with TwoProduct (fused multiply-add) and TwoSum; the
-stencil parts stay plain FP32.  The generated C lines are evaluated here by a
+stencil part (the Laplacian) stays plain FP32 and its rounded coefficient
+multiplies it as a pair.  The generated C lines are evaluated here by a
 small interpreter with numpy float32 arithmetic (every operation rounded to
 FP32, as on the device) and an exact fused multiply-add (rational arithmetic,
 one rounding).  For random Gray-Scott states near the steady balance -- where
@@ -113,8 +114,10 @@ def test_ff_increment_accuracy():
             qq = U(0, 0, fi)
             lap = ((U(1, 0, fi) + U(-1, 0, fi) - 2 * qq) + (U(0, 1, fi) + U(0, -1, fi) - 2 * qq)) / Fraction(float(1e-4))
             Ff, kf = Fraction(F), Fraction(k)
-            # the stencil part as the kernel has it (plain FP32), the rest exact
-            diff = Fraction(float(env["s%d_%d" % (kk, diff_id[fname])]))
+            # the stencil part (the Laplacian) as the kernel has it in plain FP32,
+            # times the exact (FP64) coefficient; the reaction terms exact
+            dm = node.micro[diff_id[fname]]
+            diff = Fraction(float(body.vals[dm.inst[2]])) * Fraction(float(env["s%d_%d" % (kk, dm.inst[3])]))
             if fname == "u":
                 exact = diff - u * v * v + Ff * (1 - u)
                 full = Fraction(2e-5) * lap - u * v * v + Ff * (1 - u)
@@ -132,8 +135,9 @@ def test_ff_increment_accuracy():
 
 
 def test_stencil_parts_stay_plain():
-    """The Laplacian and its multiples stay plain FP32: no pair is formed for a
-    value that reads two or more cursors through + - and constant factors."""
+    """The Laplacian stays plain FP32 (no pair for a value that reads two or more
+    cursors through + - and FP32-exact constant factors); the diffusivity, not
+    an FP32 number, multiplies it as a pair."""
     em, mm = nodes()
     kk = em.plan.stores[0]
     node = em.plan.mm[kk]
@@ -146,3 +150,5 @@ def test_stencil_parts_stay_plain():
     # the d_xx / d_yy quotients (x 10000) are read as plain values, never paired
     assert "10000" not in text and "1.000000000e+04f" not in text
     assert "fmaf(" in text and "Ph[0]" in text and "Pl[0]" in text
+    ru = np.float32(2e-5)
+    assert ("%.9ef" % float(ru)) in text and ("%.9ef" % float(np.float32(2e-5 - float(ru)))) in text
