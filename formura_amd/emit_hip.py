@@ -77,12 +77,18 @@ class LibEmitter:
         self.step = GraphPlan(step_mm, self.cfg.sleeve, self.dim, "step", storage=storage)
         self.filter = GraphPlan(mm.filter, self.cfg.filter_sleeve, self.dim, "filter") if mm.filter else None
         self.first = GraphPlan(mm.first_step, self.cfg.sleeve0, self.dim, "first_step") if mm.first_step else None
-        if self.cfg.blocking is not None:
-            for plan in (self.step,):
-                for k in plan.stores:
-                    if any(m.inst[0] == "LoadIndex" for m in plan.mm[k].micro):
-                        raise EmitError("index access in a temporally blocked step is not supported")
+        # index reads in a temporally blocked step: every sub-step of a Forward
+        # reads i + offset + block_offset with the Navi of the Forward's start
+        # (Templates.hs:373,481), which only a schedule that runs one Forward
+        # per launch over unwrapped tile coordinates reproduces (tile2d, k = nt)
+        self.tb_index = self.cfg.blocking is not None and any(
+            m.inst[0] == "LoadIndex" for k in self.step.stores for m in self.step.mm[k].micro)
         self.schedule = choose_schedule(self, schedule)
+        if self.tb_index and not (isinstance(self.schedule, Tile2DSchedule) and not self.schedule.inplace
+                                  and self.schedule.k == self.cfg.steps_per_forward):
+            raise EmitError("index access in a temporally blocked step needs the tile2d schedule with "
+                            "k = temporal_blocking_interval (%d) and ping-pong buffers"
+                            % self.cfg.steps_per_forward)
 
     # -- helpers -----------------------------------------------------------------
     def prec_of(self, node):
@@ -464,6 +470,7 @@ class Tile2DSchedule:
         body_lines = []
         params = ", ".join("const %s* __restrict__ lds%d" % (ct, i) for i, (n, ct) in enumerate(fields))
         outs = ", ".join("%s& o%d" % (ct, i) for i, (n, ct) in enumerate(fields))
+        tbi = getattr(em, "tb_index", False)
         for k in plan.stores:
             node = plan.mm[k]
             r = plan.rng[k]
@@ -477,12 +484,18 @@ class Tile2DSchedule:
             def store(fname, v):
                 return "o%d = %s;" % (em.field_index[fname], v)
 
-            b = Body(node, load, lambda *a: None, lambda ax: None, store, "s%d_" % k, prec=em.prec_of(node))
+            def index(ax):
+                # unwrapped tile coordinate + 2s(nt - j) + the Forward's offset
+                return "(%s + ib%d)" % ("gx" if ax == 0 else "gy", ax)
+
+            b = Body(node, load, lambda *a: None, index if tbi else (lambda ax: None), store, "s%d_" % k,
+                     prec=em.prec_of(node))
             body_lines.append("{")
             body_lines += ["  " + l for l in b.lines]
             body_lines.append("}")
-        out.append("template <int RY>\n__device__ __forceinline__ void pt_step(%s, const int o, %s) {\n%s\n}\n"
-                   % (params, outs, "\n".join("  " + l for l in body_lines)))
+        ipar = ", const int gx, const int gy, const int ib0, const int ib1" if tbi else ""
+        out.append("template <int RY>\n__device__ __forceinline__ void pt_step(%s, const int o%s, %s) {\n%s\n}\n"
+                   % (params, ipar, outs, "\n".join("  " + l for l in body_lines)))
         wrap1 = em.wrap_fn(0, 2 * self.k * s + 1)
         wrap2 = em.wrap_fn(1, 2 * self.k * s + 1)
         nb = 1 if self.inplace else 2
@@ -493,6 +506,10 @@ class Tile2DSchedule:
                               [("B%d[0][idx]" % i, "FLD%d(st_in)[g]" % i) for i in range(len(fields))],
                               fields[0][1], "ld", threads=self.threads)
         srcs = ", ".join("B%d[src]" % i for i in range(len(fields)))
+        if tbi:
+            srcs += ", o, x0 + r - H, y0 + c - H, ib0, ib1"
+        else:
+            srcs += ", o"
         decl_o = " ".join("%s o%d;" % (ct, i) for i, (n, ct) in enumerate(fields))
         refs_o = ", ".join("o%d" % i for i in range(len(fields)))
         st_lds = " ".join("B%d[dst][o] = o%d;" % (i, i) for i in range(len(fields)))
@@ -508,7 +525,7 @@ class Tile2DSchedule:
             return out
         out.append("""
 template <int K>
-__global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
+__global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out%(offp)s) {
   constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
   constexpr int NTX = N1 / TX, NTY = N2 / TY, NT = NTX * NTY;
 %(lds)s
@@ -522,13 +539,13 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ 
 #pragma unroll
   for (int j = 1; j <= K; ++j) {
     const int src = (j - 1) & 1, dst = j & 1;
-    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
+    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;%(ibdecl)s
     for (int idx = threadIdx.x; idx < HH * W; idx += %(NTH)d) {
       const int rr = idx / W;
       const int r = lo + rr, c = lo + (idx - rr * W);
       const int o = r * RY + c;
       %(decl_o)s
-      pt_step<RY>(%(srcs)s, o, %(refs_o)s);
+      pt_step<RY>(%(srcs)s, %(refs_o)s);
       if (j < K) {
         %(st_lds)s
       } else {
@@ -540,11 +557,22 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ 
   }
 }
 """ % dict(TX=TX, TY=TY, NTH=self.threads, lds=lds, wrap1=wrap1, wrap2=wrap2, load_tile=load_tile, decl_o=decl_o,
-           srcs=srcs, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb))
+           srcs=srcs, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb,
+           offp=", const int offT0, const int offT1" if tbi else "",
+           ibdecl=("\n    // index reads: the value at unwrapped tile coordinate g of level j of the\n"
+                   "    // Forward reads g + 2s(K - j) + the offset at the Forward's start\n"
+                   "    const int ib0 = 2 * SLEEVE * (K - j) + offT0, ib1 = 2 * SLEEVE * (K - j) + offT1;")
+           if tbi else ""))
         return out
 
     def launch(self, em, plan):
         nt = (em.N[0] // self.tx) * (em.N[1] // self.ty)
+        if getattr(em, "tb_index", False):
+            # one Forward per launch; offsets of the Navi at the Forward's start
+            return ("static void run_step_tile(int k, const char* in, char* out, hipStream_t s, int o0, int o1) {\n"
+                    "  if (k != %d) fmr_rt::fatal(\"bad step count\");\n"
+                    "  hipLaunchKernelGGL(k_step_tile<%d>, dim3(%d), dim3(%d), 0, s, in, out, o0, o1);\n"
+                    "  FMR_CHECK(hipGetLastError());\n}\n" % (self.k, self.k, nt, self.threads))
         return ("static void run_step_tile(int k, const char* in, char* out, hipStream_t s) {\n"
                 "  switch (k) {\n%s  default: fmr_rt::fatal(\"bad step count\");\n  }\n"
                 "  FMR_CHECK(hipGetLastError());\n}\n"
@@ -552,6 +580,13 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ 
                           % (k, k, nt, self.threads) for k in range(1, self.k + 1)))
 
     def advance(self, em):
+        if getattr(em, "tb_index", False):
+            offs = _offsets_at(em.om.axes, "n0", "done")
+            return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
+                    "  if (steps %% %d) fmr_rt::fatal(\"a temporally blocked step with index reads advances whole Forwards\");\n"
+                    "  for (long long done = 0; done < steps; done += %d) {\n"
+                    "    run_step_tile(%d, g_rt.cur, g_rt.nxt, g_rt.stream, %s); g_rt.swap();\n"
+                    "    g_rt.launches += 1;\n  }\n}" % (self.k, self.k, self.k, ", ".join(offs)))
         return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
                 "  (void)n0;\n"
                 "  while (steps > 0) {\n"
@@ -1258,7 +1293,9 @@ def _tile2d_ok(em, tx, ty, k, inplace=False):
     if em.dim != 2 or plan.temps:
         return False
     if any(m.inst[0] == "LoadIndex" for kk in plan.stores for m in plan.mm[kk].micro):
-        return False
+        # only the temporally blocked form, one Forward per launch (see LibEmitter)
+        if not getattr(em, "tb_index", False) or k != em.cfg.steps_per_forward or inplace:
+            return False
     if em.N[0] % tx or em.N[1] % ty:
         return False
     return _tile2d_lds(em, tx, ty, k, inplace) <= LDS_PER_WORKGROUP
@@ -1389,6 +1426,11 @@ def choose_schedule(em, want):
             if want != "auto":
                 raise
     if want in ("auto", "tile2d") and em.dim == 2:
+        if getattr(em, "tb_index", False):
+            nt = em.cfg.steps_per_forward
+            for tx, ty in ((32, 32), (16, 32), (16, 16), (8, 8)):
+                if _tile2d_ok(em, tx, ty, nt):
+                    return Tile2DSchedule(tx, ty, nt)
         for tx, ty, k in ((32, 32, 4), (16, 16, 4), (32, 32, 1), (16, 16, 1), (8, 8, 1)):
             if _tile2d_ok(em, tx, ty, k):
                 return Tile2DSchedule(tx, ty, k)

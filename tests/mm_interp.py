@@ -102,3 +102,122 @@ def init(mm_init, shape):
         else:
             temps[k] = np.array(v)
     return out
+
+
+def _eval_block(node, buff, base, out_shape, params, index):
+    """A Store node over one block of the reference's temporally blocked update:
+    output i reads buff[i + d + off(range)] (Templates.hs:473, no wrap: buff
+    carries the block's high-side walls); ``index[k]`` is the value of the
+    index expression along axis k for every output i (Templates.hs:481)."""
+    vals = []
+    params = params or {}
+    ext = {"sqrt": np.sqrt, "exp": np.exp, "fabs": np.fabs, "sin": np.sin, "cos": np.cos}
+    for m in node.micro:
+        k = m.inst[0]
+        if m.src in params and k not in ("LoadCursorStatic", "LoadCursor", "LoadIndex", "Store"):
+            v = float(params[m.src])  # a literal bound to a value (the emitter's ``params``)
+        elif k == "Imm":
+            v = float(m.inst[1])
+        elif k == "LoadCursorStatic":
+            o = [c + b for c, b in zip(m.inst[1], base)]
+            v = buff[m.inst[2]][tuple(slice(a, a + n) for a, n in zip(o, out_shape))]
+        elif k == "LoadIndex":
+            ax = m.inst[1]
+            v = index[ax].reshape([-1 if a == ax else 1 for a in range(len(out_shape))])
+        elif k == "Uniop":
+            a = vals[m.inst[2]]
+            op = m.inst[1]
+            v = -a if op == "-" else (a if op == "+" else ext[op.split("/", 1)[1]](a))
+        elif k == "Binop":
+            a, b = vals[m.inst[2]], vals[m.inst[3]]
+            with np.errstate(all="ignore"):
+                v = {"+": lambda: a + b, "-": lambda: a - b, "*": lambda: a * b, "/": lambda: np.divide(a, b),
+                     "**": lambda: np.power(a, b), "<": lambda: a < b, ">": lambda: a > b,
+                     "<=": lambda: a <= b, ">=": lambda: a >= b, "==": lambda: a == b,
+                     "!=": lambda: a != b}[m.inst[1]]()
+        elif k == "Triop":
+            v = np.where(vals[m.inst[2]], vals[m.inst[3]], vals[m.inst[4]])
+        elif k == "Store":
+            v = vals[m.inst[2]]
+        else:
+            raise NotImplementedError(k)
+        vals.append(v)
+    return vals[-1]
+
+
+def forward_tb(mm_graph, sleeve, fields, nt, gpb, offsets, params=None):
+    """One Formura_Forward of a temporally blocked program, restating
+    Templates.hs:311-381 step by step (no MPI, one rank).
+
+    tmp_floor[f] = data[(f - 2 s nt) mod N] for f in [0, N + 2 s nt) per axis
+    (copy with offset 2 s nt, then the periodic low-side halo).  Blocks of gpb
+    cells run in loop order (the first axis outermost, block index j ascending,
+    so positions descend), first the region b0, then the regions bs (:316-321);
+    block j starts at floor offset gpb (m - 1) - gpb j.  A block copies its
+    floor into rslt, then for each of the nt sub-steps: rslt -> buff, the
+    walls of the blocks above (written by them at the same sub-step) -> the
+    high side of buff, one Formura_Step with the Navi of the Forward's start
+    and the block's floor offset as block offsets (an index reads
+    i + offset + block_offset), and the low side of buff -> the walls.  rslt
+    goes back to the floor, and the floor's first N cells become the state.
+    The walls are static arrays: zero before their first write.  Programs
+    without manifest temporaries."""
+    s = sleeve
+    names = sorted(fields)
+    shape = fields[names[0]].shape
+    dim = len(shape)
+    rng = calc_range(mm_graph)
+    for k in mm_graph:
+        if mm_graph[k].micro[-1].inst[0] != "Store":
+            raise NotImplementedError("temporaries under temporal blocking")
+    h = 2 * s * nt
+    fsize = [n + h for n in shape]
+    for f in fsize:
+        assert f % gpb == 0
+    m = [f // gpb for f in fsize]
+    floor = {f: fields[f][np.ix_(*[(np.arange(fs) - h) % n for fs, n in zip(fsize, shape)])] for f in names}
+    walls = [dict() for _ in range(dim)]
+    dd = [h // gpb for _ in range(dim)]
+    b0 = [(0, m[a] - 1 - dd[a]) for a in range(dim)]
+    bs = [[((m[a] - 1 - dd[a]) if a == j else 0, (m[a] - 1 - dd[a]) if a > j else m[a]) for a in range(dim)]
+          for j in range(dim)]
+
+    def blocks(region):
+        import itertools
+        return itertools.product(*[range(l, u) for l, u in region])
+
+    def update(J):
+        fo = [gpb * (m[a] - 1) - gpb * J[a] for a in range(dim)]
+        sl = tuple(slice(o, o + gpb) for o in fo)
+        rslt = {f: floor[f][sl].copy() for f in names}
+        for it in range(nt):
+            buff = {f: np.zeros([gpb + 2 * s] * dim) for f in names}
+            for f in names:
+                buff[f][tuple(slice(0, gpb) for _ in range(dim))] = rslt[f]
+            for a in range(dim):
+                key = tuple(J[b] for b in range(dim) if b != a) + (it,)
+                region = tuple(slice(gpb, gpb + 2 * s) if b == a else slice(0, gpb + 2 * s) for b in range(dim))
+                for f in names:
+                    buff[f][region] = walls[a].get(key, {}).get(f, 0.0)
+            index = [np.arange(gpb, dtype=np.float64) + float(offsets[a] + fo[a]) for a in range(dim)]
+            new = {}
+            for k in sorted(mm_graph):
+                node = mm_graph[k]
+                base = [rng[k].offset] * dim
+                new[node.micro[-1].inst[1]] = np.array(np.broadcast_to(
+                    np.asarray(_eval_block(node, buff, base, [gpb] * dim, params, index), dtype=np.float64),
+                    [gpb] * dim))
+            rslt = new
+            for a in range(dim):
+                key = tuple(J[b] for b in range(dim) if b != a) + (it,)
+                region = tuple(slice(0, 2 * s) if b == a else slice(0, gpb + 2 * s) for b in range(dim))
+                walls[a][key] = {f: buff[f][region].copy() for f in names}
+        for f in names:
+            floor[f][sl] = rslt[f]
+
+    for J in blocks(b0):
+        update(J)
+    for region in bs:
+        for J in blocks(region):
+            update(J)
+    return {f: floor[f][tuple(slice(0, n) for n in shape)].copy() for f in names}
