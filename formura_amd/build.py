@@ -59,6 +59,7 @@ PROGRAMS: Dict[str, tuple] = {
     "euler3d_32_staged": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "staged3d", "ty": 16, "tz": 16, "cx": 8, "threads": 512}),
     "index2d_32_tb": ("index2d.fmr", "index2d_32_tb.yaml", {"kind": "tile2d", "tx": 16, "ty": 16, "k": 2}),
     "index2d_32_tb8": ("index2d.fmr", "index2d_32_tb.yaml", {"kind": "tile2d", "tx": 8, "ty": 32, "k": 2, "threads": 128}),
+    "index3d_16_tb": ("index3d.fmr", "index3d_16_tb.yaml", {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 2}),
     "gs_64_tile_k8": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 8}),
     "gs_64_tile_k5_inplace": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 5, "inplace": 1}),
     "gs_64_wave_k3": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "wave2d", "rows": 16, "k": 3}),

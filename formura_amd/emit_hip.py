@@ -84,11 +84,12 @@ class LibEmitter:
         self.tb_index = self.cfg.blocking is not None and any(
             m.inst[0] == "LoadIndex" for k in self.step.stores for m in self.step.mm[k].micro)
         self.schedule = choose_schedule(self, schedule)
-        if self.tb_index and not (isinstance(self.schedule, Tile2DSchedule) and not self.schedule.inplace
-                                  and self.schedule.k == self.cfg.steps_per_forward):
-            raise EmitError("index access in a temporally blocked step needs the tile2d schedule with "
-                            "k = temporal_blocking_interval (%d) and ping-pong buffers"
-                            % self.cfg.steps_per_forward)
+        if self.tb_index and not (
+                (isinstance(self.schedule, Tile2DSchedule) and not self.schedule.inplace
+                 or isinstance(self.schedule, Stream3DSchedule))
+                and self.schedule.k == self.cfg.steps_per_forward):
+            raise EmitError("index access in a temporally blocked step needs the tile2d (ping-pong) or stream3d "
+                            "schedule with k = temporal_blocking_interval (%d)" % self.cfg.steps_per_forward)
 
     # -- helpers -----------------------------------------------------------------
     def prec_of(self, node):
@@ -1146,6 +1147,7 @@ class Stream3DSchedule:
         params = ", ".join("const %s* __restrict__ lds%d" % (ct, i) for i, (n, ct) in enumerate(fields))
         outs = ", ".join("%s& o%d" % (ct, i) for i, (n, ct) in enumerate(fields))
         body_lines = []
+        tbi = getattr(em, "tb_index", False)
         for k in plan.stores:
             node = plan.mm[k]
             base = plan.rng[k].offset - 2 * s
@@ -1158,12 +1160,18 @@ class Stream3DSchedule:
             def store(fname, v):
                 return "o%d = %s;" % (em.field_index[fname], v)
 
-            b = Body(node, load, lambda *a: None, lambda ax: None, store, "s%d_" % k, prec=em.prec_of(node))
+            def index(ax):
+                # unwrapped coordinate + 2s(nt - j) + the offset at the Forward's start
+                return "(g%d + ib%d)" % (ax, ax)
+
+            b = Body(node, load, lambda *a: None, index if tbi else (lambda ax: None), store, "s%d_" % k,
+                     prec=em.prec_of(node))
             body_lines.append("{")
             body_lines += ["  " + l for l in b.lines]
             body_lines.append("}")
-        out = ["template <int RZ>\n__device__ __forceinline__ void pt_step3(%s, const int* pl, const int o, %s) {\n%s\n}\n"
-               % (params, outs, "\n".join("  " + l for l in body_lines))]
+        ipar = ", const int g0, const int g1, const int g2, const int ib0, const int ib1, const int ib2" if tbi else ""
+        out = ["template <int RZ>\n__device__ __forceinline__ void pt_step3(%s, const int* pl, const int o%s, %s) {\n%s\n}\n"
+               % (params, ipar, outs, "\n".join("  " + l for l in body_lines))]
         nf = len(fields)
         lds = "\n".join("  __shared__ %s B%d[K][R * PLANE];" % (ct, i) for i, (n, ct) in enumerate(fields))
         wrap = [em.wrap_fn(a, 2 * self.k * s + 1) for a in range(3)]
@@ -1179,7 +1187,7 @@ class Stream3DSchedule:
         st_glb = " ".join("FLD%d(st_out)[g] = o%d;" % (i, i) for i in range(nf))
         out.append("""
 template <int K>
-__global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict__ st_in, char* __restrict__ st_out) {
+__global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict__ st_in, char* __restrict__ st_out%(offp)s) {
   constexpr int TY = %(TY)d, TZ = %(TZ)d, CX = %(CX)d, R = %(R)d;
   constexpr int H = 2 * K * SLEEVE, RY = TY + H, RZ = TZ + H, PLANE = RY * RZ;
   constexpr int NTY = N2 / TY, NTZ = N3 / TZ, NCX = N1 / CX, NT = NTY * NTZ * NCX;
@@ -1222,13 +1230,13 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict_
         int pl[R];
 #pragma unroll
         for (int d = 0; d < R; ++d) pl[d] = ((((t - 2 * SLEEVE + d) %% R) + R) %% R) * PLANE;
-        const int lo = 2 * j * SLEEVE, W = RZ - lo, HH = RY - lo;
+        const int lo = 2 * j * SLEEVE, W = RZ - lo, HH = RY - lo;%(ibdecl)s
         for (int idx = threadIdx.x; idx < HH * W; idx += %(NTH)d) {
           const int rr = idx / W;
           const int ly = lo + rr, lz = lo + (idx - rr * W);
           const int o = ly * RZ + lz;
           %(decl_o)s
-          pt_step3<RZ>(%(srcs)s, pl, o, %(refs_o)s);
+          pt_step3<RZ>(%(srcs)s, pl, o%(ipass)s, %(refs_o)s);
           if (j < K) {
             %(st_lds)s
           } else {
@@ -1242,11 +1250,22 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict_
   }
 }
 """ % dict(TY=self.ty, TZ=self.tz, CX=self.cx, R=R, NTH=self.threads, lds=lds, w0=wrap[0], w1=wrap[1], w2=wrap[2],
-           pf_decl=pf_decl, pf_load=pf_load, pf_store=pf_store, srcs=srcs, decl_o=decl_o, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb))
+           pf_decl=pf_decl, pf_load=pf_load, pf_store=pf_store, srcs=srcs, decl_o=decl_o, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb,
+           offp=", const int offT0, const int offT1, const int offT2" if tbi else "",
+           ibdecl=("\n        // index reads: level j of the Forward at unwrapped coordinate g reads\n"
+                   "        // g + 2s(K - j) + the offset at the Forward's start\n"
+                   "        const int ib0 = 2 * SLEEVE * (K - j) + offT0, ib1 = 2 * SLEEVE * (K - j) + offT1, "
+                   "ib2 = 2 * SLEEVE * (K - j) + offT2;") if tbi else "",
+           ipass=", t, y0 + ly - H, z0 + lz - H, ib0, ib1, ib2" if tbi else ""))
         return out
 
     def launch(self, em, plan):
         nt = (em.N[0] // self.cx) * (em.N[1] // self.ty) * (em.N[2] // self.tz)
+        if getattr(em, "tb_index", False):
+            return ("static void run_step_stream(int k, const char* in, char* out, hipStream_t s, int o0, int o1, int o2) {\n"
+                    "  if (k != %d) fmr_rt::fatal(\"bad step count\");\n"
+                    "  hipLaunchKernelGGL(k_step_stream<%d>, dim3(%d), dim3(%d), 0, s, in, out, o0, o1, o2);\n"
+                    "  FMR_CHECK(hipGetLastError());\n}\n" % (self.k, self.k, nt, self.threads))
         return ("static void run_step_stream(int k, const char* in, char* out, hipStream_t s) {\n"
                 "  switch (k) {\n%s  default: fmr_rt::fatal(\"bad step count\");\n  }\n"
                 "  FMR_CHECK(hipGetLastError());\n}\n"
@@ -1254,6 +1273,13 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict_
                           % (k, k, nt, self.threads) for k in range(1, self.k + 1)))
 
     def advance(self, em):
+        if getattr(em, "tb_index", False):
+            offs = _offsets_at(em.om.axes, "n0", "done")
+            return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
+                    "  if (steps %% %d) fmr_rt::fatal(\"a temporally blocked step with index reads advances whole Forwards\");\n"
+                    "  for (long long done = 0; done < steps; done += %d) {\n"
+                    "    run_step_stream(%d, g_rt.cur, g_rt.nxt, g_rt.stream, %s); g_rt.swap();\n"
+                    "    g_rt.launches += 1;\n  }\n}" % (self.k, self.k, self.k, ", ".join(offs)))
         return ("static void fmr_advance(long long steps, const Formura_Navi &n0) {\n"
                 "  (void)n0;\n"
                 "  while (steps > 0) {\n"
@@ -1267,7 +1293,9 @@ def _stream3d_ok(em, ty, tz, cx, k):
     if em.dim != 3 or plan.temps:
         return False
     if any(m.inst[0] == "LoadIndex" for kk in plan.stores for m in plan.mm[kk].micro):
-        return False
+        # only the temporally blocked form, one Forward per launch (see LibEmitter)
+        if not getattr(em, "tb_index", False) or k != em.cfg.steps_per_forward:
+            return False
     if em.N[0] % cx or em.N[1] % ty or em.N[2] % tz or cx < 1:
         return False
     s = plan.sleeve
@@ -1434,6 +1462,11 @@ def choose_schedule(em, want):
         for tx, ty, k in ((32, 32, 4), (16, 16, 4), (32, 32, 1), (16, 16, 1), (8, 8, 1)):
             if _tile2d_ok(em, tx, ty, k):
                 return Tile2DSchedule(tx, ty, k)
+    if want in ("auto", "stream3d") and em.dim == 3 and getattr(em, "tb_index", False):
+        nt = em.cfg.steps_per_forward
+        for ty, tz, cx in ((8, 32, 16), (8, 16, 8), (4, 8, 8), (4, 4, 4)):
+            if _stream3d_ok(em, ty, tz, cx, nt):
+                return Stream3DSchedule(ty, tz, cx, nt)
     if want == "stream3d" and em.dim == 3:
         for c in ((16, 64, 32, 1), (8, 32, 16, 1), (4, 8, 8, 1)):
             if _stream3d_ok(em, *c):

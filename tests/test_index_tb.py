@@ -13,8 +13,9 @@ final cell G holds the start value of cell G - s nt plus c times the indices
 G + s (nt - 1 - it) + offset of each sub-step it, and (3) equality with an
 independent statement of the GPU scheme: level j of a Forward over unwrapped
 coordinates g in [-2s(nt - j), N), an index reading g + 2s(nt - j) + offset.
-On the GPU the tile2d kernel with k = nt (one Forward per launch) gives the
-restatement bit for bit, state and Navi, on two tile shapes.
+On the GPU the tile2d kernel (2-D, two tile shapes) and the stream3d kernel
+(3-D) with k = nt (one Forward per launch) give the restatement bit for bit,
+state and Navi.
 """
 import os
 
@@ -63,14 +64,16 @@ def test_restatement_closed_form_without_diffusion(off):
 
 def _trapezoid(g, q, offT):
     rng = calc_range(g)
-    L = {f: v[np.ix_(*[np.arange(-2 * S * NT, N) % N] * 2)] for f, v in q.items()}
+    n = next(iter(q.values())).shape[0]
+    dim = len(offT)
+    L = {f: v[np.ix_(*[np.arange(-2 * S * NT, n) % n] * dim)] for f, v in q.items()}
     for j in range(1, NT + 1):
-        size = N + 2 * S * (NT - j)
+        size = n + 2 * S * (NT - j)
         # local index l is g = l - 2s(nt - j); the index reads g + 2s(nt - j) + offset = l + offset
-        index = [np.arange(size, dtype=np.float64) + float(offT[a]) for a in range(2)]
+        index = [np.arange(size, dtype=np.float64) + float(offT[a]) for a in range(dim)]
         L = {g[k].micro[-1].inst[1]: np.array(np.broadcast_to(np.asarray(
-            mm_interp._eval_block(g[k], L, [rng[k].offset] * 2, [size] * 2, None, index), dtype=np.float64),
-            [size] * 2)) for k in sorted(g)}
+            mm_interp._eval_block(g[k], L, [rng[k].offset] * dim, [size] * dim, None, index), dtype=np.float64),
+            [size] * dim)) for k in sorted(g)}
     return L
 
 
@@ -83,6 +86,22 @@ def test_restatement_equals_the_unwrapped_level_scheme():
         b = _trapezoid(g, b, off)
         np.testing.assert_array_equal(a["q"], b["q"])
         off = [(o - S * NT) % N for o in off]
+
+
+def program3d():
+    return generate(os.path.join(B.PROG_DIR, "index3d.fmr"), os.path.join(B.PROG_DIR, "index3d_16.yaml"),
+                    "index3d", "flat")[2]
+
+
+def test_restatement_equals_the_unwrapped_level_scheme_3d():
+    g = program3d().step
+    q = {"q": np.random.default_rng(5).random((16, 16, 16))}
+    a, b, off = dict(q), dict(q), [0, 0, 0]
+    for fw in range(3):
+        a = mm_interp.forward_tb(g, S, a, NT, 10, off)
+        b = _trapezoid(g, b, off)
+        np.testing.assert_array_equal(a["q"], b["q"])
+        off = [(o - S * NT) % 16 for o in off]
 
 
 def test_schedules_that_cannot_reproduce_it_are_refused():
@@ -120,3 +139,31 @@ def test_index_tb_bit_exact(name, forwards, resident):
     if not np.array_equal(got, q["q"]):
         raise AssertionError("%s differs after %d forwards: max |diff| %g"
                              % (name, forwards, float(np.max(np.abs(got - q["q"])))))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("forwards,resident", [(1, False), (4, True)])
+def test_index_tb_bit_exact_3d(forwards, resident):
+    print("index3d_16_tb: %d forwards, resident=%s" % (forwards, resident), flush=True)
+    g = program3d().step
+    h = hip_lib("index3d_16_tb")
+    h.init()
+    q = {"q": h.field("q").copy()}
+    off = [h.navi_dict()["offset_" + a] for a in "xyz"]
+    if resident:
+        h.set_resident(True)
+        h.upload()
+        h.forward_steps(forwards)
+        h.download()
+        h.set_resident(False)
+    else:
+        h.forward(forwards)
+    for _ in range(forwards):
+        q = mm_interp.forward_tb(g, S, q, NT, 10, off)
+        off = [(o - S * NT) % 16 for o in off]
+    nv = h.navi_dict()
+    assert [nv["offset_" + a] for a in "xyz"] == off and nv["time_step"] == NT * forwards
+    got = h.field("q")
+    if not np.array_equal(got, q["q"]):
+        raise AssertionError("index3d_16_tb differs after %d forwards: max |diff| %g"
+                             % (forwards, float(np.max(np.abs(got - q["q"])))))
