@@ -120,3 +120,34 @@ def test_hooks_bit_exact(forwards, resident):
     if not np.array_equal(got, r.q["q"]):
         raise AssertionError("hooks2d_32 differs after %d forwards: max |diff| %g"
                              % (forwards, float(np.max(np.abs(got - r.q["q"])))))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("forwards", [1, 5])
+def test_hooks_under_temporal_blocking_bit_exact(forwards):
+    """The same program with temporal_blocking_interval 3 and filter_interval 6
+    (hooks2d_32_tb.yaml): a Forward advances 3 steps (offsets -= 3 s at its
+    end, Templates.hs:325-328), then runs the filter when time_step, before
+    the += 3, is a multiple of 6 (:329-333).  The step has no index reads, so
+    the blocked update equals 3 plain steps (checked for the restatement in
+    tests/test_index_tb.py); the filter reads the offsets after the shift."""
+    print("hooks2d_32_tb: %d forwards" % forwards, flush=True)
+    fmr, yml, _ = B.PROGRAMS["hooks2d_32_tb"]
+    mm = generate(os.path.join(B.PROG_DIR, fmr), os.path.join(B.PROG_DIR, "hooks2d_32.yaml"), NAME, "flat")[2]
+    h = hip_lib("hooks2d_32_tb")
+    h.init()
+    r = Restatement(mm)
+    np.testing.assert_array_equal(h.field("q"), r.q["q"])
+    filters = []
+    for _ in range(forwards):
+        for _ in range(3):
+            r._apply(mm.step, r.s)
+        if r.time_step % 6 == 0:
+            filters.append(r.off[0])
+            r._apply(mm.filter, r.sf)
+        r.time_step += 3
+    h.forward(forwards)
+    nv = h.navi_dict()
+    assert (nv["time_step"], nv["offset_x"], nv["offset_y"]) == (r.time_step, r.off[0], r.off[1])
+    assert len(filters) == (forwards + 1) // 2
+    np.testing.assert_array_equal(h.field("q"), r.q["q"])
