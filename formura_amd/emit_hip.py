@@ -144,16 +144,20 @@ class LibEmitter:
 
             return load, tmp_load
 
-        for k in plan.temps:
-            node = plan.mm[k]
+        for gi, grp in enumerate(temp_groups(plan)):
+            # the temporaries of one dependency level, as one kernel: inputs read
+            # once, common instructions shared (same values as one kernel each)
             load, tmp_load = loaders([0] * dim)
-            body = Body(node, load, tmp_load, lambda ax: "0", lambda n, v: "", "a", prec=self.prec_of(node))
-            body.lines.append("t_%d[t] = %s;" % (k, body.result))
-            deps = [d for d in plan.temps if d < k]
+            precs = [self.prec_of(plan.mm[k]) or {} for k in grp] if self.mixed else None
+            node, roots, mprec = _merge_nodes([plan.mm[k] for k in grp], precs)
+            body = Body(node, load, tmp_load, lambda ax: "0", lambda n, v: "", "a", prec=mprec)
+            body.lines += ["t_%d[t] = (%s)a%d;" % (k, plan.temp_ctype(k), roots[i]) for i, k in enumerate(grp)]
+            deps = temp_group_inputs(plan, grp)
+            outs_ = "".join(", %s* __restrict__ t_%d" % (plan.temp_ctype(k), k) for k in grp)
             params = "".join(", const %s* __restrict__ t_%d" % (plan.temp_ctype(d), d) for d in deps)
             out.append(
-                "__global__ void __launch_bounds__(256) k_%s_tmp%d(const char* __restrict__ st_in, %s* __restrict__ t_%d%s) {\n"
-                % (plan.tag, k, plan.temp_ctype(k), k, params)
+                "__global__ void __launch_bounds__(256) k_%s_tmpg%d(const char* __restrict__ st_in%s%s) {\n"
+                % (plan.tag, gi, outs_, params)
                 + "\n".join("  " + l for l in decompose() + body.lines) + "\n}\n")
         # one fused kernel for every Store node
         lines = decompose()
@@ -185,10 +189,11 @@ class LibEmitter:
         dim = self.dim
         L = ["static void run_%s(const char* in, char* out, const int* offs, hipStream_t s) {" % plan.tag,
              "  const unsigned nb = (unsigned)((NTOT + 255) / 256);"]
-        for k in plan.temps:
-            deps = [d for d in plan.temps if d < k]
-            L.append("  hipLaunchKernelGGL(k_%s_tmp%d, dim3(nb), dim3(256), 0, s, in, (%s*)g_tmp_%s[%d]%s);"
-                     % (plan.tag, k, plan.temp_ctype(k), plan.tag, plan.temps.index(k),
+        for gi, grp in enumerate(temp_groups(plan)):
+            deps = temp_group_inputs(plan, grp)
+            L.append("  hipLaunchKernelGGL(k_%s_tmpg%d, dim3(nb), dim3(256), 0, s, in%s%s);"
+                     % (plan.tag, gi,
+                        "".join(", (%s*)g_tmp_%s[%d]" % (plan.temp_ctype(k), plan.tag, plan.temps.index(k)) for k in grp),
                         "".join(", (const %s*)g_tmp_%s[%d]" % (plan.temp_ctype(d), plan.tag, plan.temps.index(d)) for d in deps)))
         args = "".join(", (const %s*)g_tmp_%s[%d]" % (plan.temp_ctype(d), plan.tag, plan.temps.index(d)) for d in plan.temps)
         offs = "".join(", offs[%d]" % k for k in range(dim))
@@ -406,6 +411,45 @@ extern "C" long long Formura_HIP_Positivity_Violations(void) { return fmr_rt::vi
         return "\n".join(L)
 
 
+TEMP_GROUP_MICRO = 20000  # instructions of one merged temporary kernel (before sharing): one kernel per level in practice
+
+
+def temp_groups(plan):
+    """Manifest temporaries in launch order, grouped: by dependency level (a
+    temporary's level is one more than the highest level it reads; state-only
+    ones are level 1), each level cut in id order into groups of at most
+    TEMP_GROUP_MICRO instructions.  A group reads only lower levels
+    (``temp_group_inputs``)."""
+    level = {}
+    for k in plan.temps:
+        deps = [m.inst[2] for m in plan.mm[k].micro if m.inst[0] == "LoadCursor"]
+        level[k] = 1 + max([level[d] for d in deps] + [0])
+    out = []
+    for lv in sorted(set(level.values())):
+        cur, size = [], 0
+        for k in plan.temps:
+            if level[k] != lv:
+                continue
+            n = len(plan.mm[k].micro)
+            if cur and size + n > TEMP_GROUP_MICRO:
+                out.append(cur)
+                cur, size = [], 0
+            cur.append(k)
+            size += n
+        if cur:
+            out.append(cur)
+    return out
+
+
+def temp_group_inputs(plan, grp):
+    """The temporaries a group may read: every one of a lower level."""
+    level = {}
+    for k in plan.temps:
+        deps = [m.inst[2] for m in plan.mm[k].micro if m.inst[0] == "LoadCursor"]
+        level[k] = 1 + max([level[d] for d in deps] + [0])
+    return [d for d in plan.temps if level[d] < level[grp[0]]]
+
+
 def _is_void(t):
     return isinstance(t, S.ElemType) and t.name == "void"
 
@@ -435,7 +479,7 @@ class FlatSchedule:
                 "  for (long long t = 0; t < steps; ++t) {\n"
                 "    const int offs[] = {%s};\n"
                 "    run_step(g_rt.cur, g_rt.nxt, offs, g_rt.stream); g_rt.swap();\n"
-                "    g_rt.launches += %d;\n  }\n}" % (", ".join(offs), len(em.step.temps) + 1))
+                "    g_rt.launches += %d;\n  }\n}" % (", ".join(offs), len(temp_groups(em.step)) + 1))
 
 
 class Tile2DSchedule:
@@ -858,8 +902,8 @@ def _merge_nodes(nodes, precs=None):
                 inst = (k, inst[1], remap[inst[2]], remap[inst[3]])
             elif k == "Triop":
                 inst = (k, inst[1], remap[inst[2]], remap[inst[3]], remap[inst[4]])
-            elif k not in ("Imm", "LoadCursorStatic"):
-                raise EmitError("staged3d: temporaries may read only the state (found %s)" % k)
+            elif k not in ("Imm", "LoadCursorStatic", "LoadCursor"):
+                raise EmitError("cannot merge the instruction %s" % k)
             p = pr.get(j) if pr else None
             key = (inst, m.typ, m.src if k == "Imm" else None, p)
             if key not in memo:
