@@ -153,6 +153,48 @@ def run_ensemble(count: int, steps: int):
     return out
 
 
+def run_fp32_single(steps: int = 10000, seed: int = 7):
+    """The headline workload (Gray-Scott 2048^2, F = 0.04, k = 0.06, the same
+    seeded state) for 10^4 steps in the two FP32 forms -- f32k (Kahan-compensated
+    accumulation) and f32c (and the increment in float-float FP32) -- and in FP64,
+    each as a one-instance batch (the FP64 batch is bit-identical to the plain
+    path at the program's own parameters): GLUPS and the error of each FP32 run
+    against FP64 per field (relative L2, maximum absolute, relative maximum), and
+    the same for FP64 from the state with u perturbed at FP32 rounding scale
+    (how far this run amplifies a rounding-size change of its initial state)."""
+    import numpy as np
+    from formura_amd import batch as B
+    st = B.grayscott_init(2048, seed)
+    params = np.array([[0.04, 0.06]])
+    out = {"grid": [2048, 2048], "steps": steps}
+    res = {}
+    for name in ("gs_batch_f64", "gs_batch_f32k", "gs_batch_f32c"):
+        e = B.Ensemble(name, [(2048, 2048)], params)
+        e.set_state(0, st)
+        e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"])  # warm-up, captures the graph
+        e.set_state(0, st)
+        ms = e.step(steps)
+        res[name] = e.get_state(0)
+        out[name] = {"glups": 2048 * 2048 * steps / (ms * 1e-3) / 1e9, "ms": ms}
+        e.close()
+    # control: FP64 again from the state with u perturbed at FP32 rounding scale
+    from formura_amd import inputs
+    e = B.Ensemble("gs_batch_f64", [(2048, 2048)], params)
+    r = inputs.uniform(977, st["u"].shape)
+    e.set_state(0, {"u": st["u"] * (1.0 + 2.0 ** -24 * (2.0 * r - 1.0)), "v": st["v"]})
+    e.step(steps)
+    res["control"] = e.get_state(0)
+    e.close()
+    x = res["gs_batch_f64"]
+    for prec in ("f32k", "f32c", "control"):
+        y = res["gs_batch_" + prec] if prec != "control" else res["control"]
+        out[prec + "_error_vs_fp64" if prec != "control" else "f64_perturbed_2^-24_vs_fp64"] = {
+            f: {"rel_l2": float(np.sqrt(np.sum((y[f] - x[f]) ** 2) / np.sum(x[f] ** 2))),
+                "max_abs": float(np.max(np.abs(y[f] - x[f]))),
+                "rel_max": float(np.max(np.abs(y[f] - x[f])) / np.max(np.abs(x[f])))} for f in ("u", "v")}
+    return out
+
+
 def run_per_instance_fp64(sizes, steps: int, probe_steps: int = 600):
     """The plain per-instance FP64 GPU path for the same ensemble: every instance
     advanced by its own plain Formura_Forward library (state resident, one
@@ -441,9 +483,10 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the heat 512^3, Euler 256^3 and adjoint legs")
-    ap.add_argument("--legs", default="plain,ensemble,heat,euler,adjoint,samples,cpu",
+    ap.add_argument("--legs", default="plain,fp32,ensemble,heat,euler,adjoint,samples,cpu",
                     help="comma-separated legs to run (for profiling one kernel family): plain (Gray-Scott "
-                         "2048^2, the headline value), ensemble, heat, euler, adjoint, samples, cpu; "
+                         "2048^2, the headline value), fp32 (the same in the FP32 forms, error vs FP64), "
+                         "ensemble, heat, euler, adjoint, samples, cpu; "
                          "adjoint_probe (a short adjoint run for profiling, not in the default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB, 20 times (the reference "
@@ -479,7 +522,8 @@ def main():
         legs.discard("ensemble")
     if "plain" not in legs:
         line = {"legs": sorted(legs)}
-        for key, leg, fn in (("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
+        for key, leg, fn in (("grayscott2048_fp32", "fp32", run_fp32_single),
+                             ("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
                              ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples),
                              ("adjoint_probe", "adjoint_probe", run_adjoint_probe)):
             if leg in legs:
@@ -527,7 +571,8 @@ def main():
             line["ensemble"] = ens
         except Exception as e:
             line["ensemble"] = {"error": str(e)[:300]}
-    for key, leg, fn in (("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
+    for key, leg, fn in (("grayscott2048_fp32", "fp32", run_fp32_single),
+                         ("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
                          ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples)):
         if leg in legs:
             try:
