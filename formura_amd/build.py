@@ -41,6 +41,8 @@ PROGRAMS: Dict[str, tuple] = {
     "heat3d_512": ("heat3d.fmr", "heat3d_512.yaml", "auto"),
     "gs_64": ("grayscott2d.fmr", "gs_64.yaml", "auto"),
     "hooks2d_32": ("hooks2d.fmr", "hooks2d_32.yaml", "auto"),
+    "float2d_32": ("float2d.fmr", "float2d_32.yaml", "flat"),
+    "float2d_32_tile": ("float2d.fmr", "float2d_32.yaml", {"kind": "tile2d", "tx": 16, "ty": 16, "k": 3}),
     "hooks2d_32_tb": ("hooks2d.fmr", "hooks2d_32_tb.yaml", "auto"),
     "gs_1024": ("grayscott2d.fmr", "gs_1024.yaml", "auto"),
     "gs_2048": ("grayscott2d.fmr", "gs_2048.yaml", "auto"),

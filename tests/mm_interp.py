@@ -221,3 +221,52 @@ def forward_tb(mm_graph, sleeve, fields, nt, gpb, offsets, params=None):
         for J in blocks(region):
             update(J)
     return {f: floor[f][tuple(slice(0, n) for n in shape)].copy() for f in names}
+
+
+_NP_TYPE = {"float": np.float32, "double": np.float64, "Rational": np.float64}
+
+
+def step_typed(mm_graph, sleeve, fields, offsets=None):
+    """One step of a program with FP32 (``float``) values, in the C types of the
+    generated code (Templates.hs:459-467): every micro value is held in its
+    declared type; an operation on two floats is done in float, one with a
+    double operand in double (the usual arithmetic conversions), and the result
+    is converted to the declared type on assignment.  Stores only (no
+    temporaries); index reads as in ``step``."""
+    rng = calc_range(mm_graph)
+    shape = next(iter(fields.values())).shape
+    out = {}
+    for k in sorted(mm_graph):
+        node = mm_graph[k]
+        if node.micro[-1].inst[0] != "Store":
+            raise NotImplementedError("temporaries")
+        base = [rng[k].offset - 2 * sleeve] * len(shape)
+        vals, types = [], []
+        for m in node.micro:
+            kk = m.inst[0]
+            t = _NP_TYPE.get(str(m.typ), np.float64)
+            if kk == "Imm":
+                v = np.float64(float(m.inst[1]))
+            elif kk == "LoadCursorStatic":
+                v = _shift(fields[m.inst[2]], [c + b for c, b in zip(m.inst[1], base)])
+            elif kk == "LoadIndex":
+                ax = m.inst[1]
+                ix = np.arange(shape[ax], dtype=np.float64) + float((offsets or [0] * len(shape))[ax])
+                v = ix.reshape([-1 if a == ax else 1 for a in range(len(shape))])
+            elif kk == "Uniop" and m.inst[1] == "-":
+                v = -vals[m.inst[2]]
+            elif kk == "Binop":
+                a, b = vals[m.inst[2]], vals[m.inst[3]]
+                wide = np.float64 if np.float64 in (types[m.inst[2]], types[m.inst[3]]) else np.float32
+                a, b = np.asarray(a, dtype=wide), np.asarray(b, dtype=wide)
+                v = {"+": a + b, "-": a - b, "*": a * b, "/": a / b}[m.inst[1]]
+            elif kk == "Store":
+                out[m.inst[1]] = np.array(np.broadcast_to(vals[m.inst[2]], shape)).astype(fields[m.inst[1]].dtype)
+                vals.append(None)
+                types.append(None)
+                continue
+            else:
+                raise NotImplementedError(kk)
+            vals.append(np.asarray(v, dtype=t))
+            types.append(t)
+    return out

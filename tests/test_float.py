@@ -1,0 +1,72 @@
+"""Programs on FP32 state (``float`` fields and values).
+
+The generated C gives every micro value its declared type (Templates.hs:459-467)
+and C's usual arithmetic conversions apply: float + float is a float sum,
+float * double (a literal) is a double product rounded to float on assignment.
+``mm_interp.step_typed`` restates that with numpy and is pinned here by a hand
+transcription of the step of ``float2d.fmr`` in the same types; the GPU
+libraries (flat and tile2d, a 32 x 48 grid) must give it bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import mm_interp
+from conftest import hip_lib
+from formura_amd import build as B
+from formura_amd.emit_hip import generate
+
+
+def program():
+    return generate(os.path.join(B.PROG_DIR, "float2d.fmr"), os.path.join(B.PROG_DIR, "float2d_32.yaml"),
+                    "float2d", "flat")[2]
+
+
+def hand_step(q):
+    """q + r*(q[i+1,j] + q[i-1,j] + q[i,j+1] + q[i,j-1] - 4*q) with float loads
+    and sums, the double literals 4 and r = 1/10 (products in double, rounded
+    to float), in the drifting frame: output i reads input i - 1 at the centre."""
+    f32 = np.float32
+    c = np.roll(q, (1, 1), axis=(0, 1))          # q[i-1, j-1]: the centre
+    xp = np.roll(q, (0, 1), axis=(0, 1))         # q[i, j-1]
+    xm = np.roll(q, (2, 1), axis=(0, 1))         # q[i-2, j-1]
+    yp = np.roll(q, (1, 0), axis=(0, 1))
+    ym = np.roll(q, (1, 2), axis=(0, 1))
+    s = ((xp + xm) + yp) + ym                    # float sums
+    four_c = (4.0 * c.astype(np.float64)).astype(f32)
+    d = s - four_c
+    rd = (0.1 * d.astype(np.float64)).astype(f32)
+    return (c + rd).astype(f32)
+
+
+def test_typed_interpreter_matches_hand_transcription():
+    mm = program()
+    q = np.random.default_rng(4).random((32, 48)).astype(np.float32)
+    a, b = {"q": q}, q
+    for _ in range(5):
+        a = mm_interp.step_typed(mm.step, 1, a)
+        b = hand_step(b)
+        assert a["q"].dtype == np.float32
+        np.testing.assert_array_equal(a["q"], b)
+    # and the double-precision reading of the same graph differs (the types matter)
+    d = mm_interp.step(mm.step, 1, {"q": q.astype(np.float64)})["q"]
+    assert not np.array_equal(d.astype(np.float32), hand_step(q))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,steps", [("float2d_32", 7), ("float2d_32_tile", 10)])
+def test_float_library_bit_exact(name, steps):
+    print("%s: %d steps" % (name, steps), flush=True)
+    mm = program()
+    h = hip_lib(name)
+    h.init()
+    q = {"q": h.field("q").copy()}
+    assert q["q"].dtype == np.float32 and q["q"].shape == (32, 48)
+    rng = np.random.default_rng(9)
+    q = {"q": (q["q"] + rng.random(q["q"].shape).astype(np.float32)).astype(np.float32)}
+    h.set_state(q)
+    h.forward(steps)
+    for _ in range(steps):
+        q = mm_interp.step_typed(mm.step, 1, q)
+    np.testing.assert_array_equal(h.field("q"), q["q"])
