@@ -1256,8 +1256,10 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict_
     }
   };
   prefetch(x0 - H);
+  // ring slot of plane t, carried along (no division in the plane loop)
+  int rs = (((x0 - H) %% R) + R) %% R;
   for (int t = x0 - H; t < x0 + CX; ++t) {
-    const int slot = ((t %% R) + R) %% R;
+    const int slot = rs;
     // every read of this ring slot's previous plane ended before the last barrier
 #pragma unroll
     for (int it = 0; it < PIT; ++it) {
@@ -1273,7 +1275,10 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict_
       if (t >= x0 - 2 * (K - j) * SLEEVE) {
         int pl[R];
 #pragma unroll
-        for (int d = 0; d < R; ++d) pl[d] = ((((t - 2 * SLEEVE + d) %% R) + R) %% R) * PLANE;
+        for (int d = 0; d < R; ++d) {  // slot of plane t - 2s + d
+          const int q = rs + d - 2 * SLEEVE;
+          pl[d] = (q < 0 ? q + R : q) * PLANE;
+        }
         const int lo = 2 * j * SLEEVE, W = RZ - lo, HH = RY - lo;%(ibdecl)s
         for (int idx = threadIdx.x; idx < HH * W; idx += %(NTH)d) {
           const int rr = idx / W;
@@ -1291,6 +1296,7 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict_
       }
       __syncthreads();
     }
+    rs = rs + 1 == R ? 0 : rs + 1;
   }
 }
 """ % dict(TY=self.ty, TZ=self.tz, CX=self.cx, R=R, NTH=self.threads, lds=lds, w0=wrap[0], w1=wrap[1], w2=wrap[2],
