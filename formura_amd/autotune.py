@@ -1,6 +1,7 @@
 """Autotuner for the kernel schedule of a program (tile shape, steps per launch).
 
-Two phases, so that compilation stays off the GPU box:
+Two phases, so that compilation runs on a host without a GPU and only the
+timing needs one:
 
 ``python -m formura_amd.autotune build NAME...``
     generates and compiles one library per candidate schedule
@@ -115,7 +116,7 @@ def run_variants(name: str, out_dir: str, seconds: float = 0.3, seed: int = 5):
             same = True
         else:
             same = all(np.array_equal(st[f], ref_state[f]) for f in st)
-        glups = cells * (steps + min(steps, 48)) / 1e9 / 1 if False else cells * steps / (ms * 1e-3) / 1e9
+        glups = cells * steps / (ms * 1e-3) / 1e9
         results.append({"spec": spec, "tag": d, "glups": glups, "ms": ms, "steps": steps,
                         "launches": lib.last_launches(), "identical_to_first": bool(same)})
         print("%-40s %8.2f GLUPS  %s" % (d, glups, "ok" if same else "MISMATCH"), flush=True)
@@ -204,7 +205,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("phase", choices=["build", "run"])
     ap.add_argument("names", nargs="+")
-    ap.add_argument("--out", default="gpurun_out/tuning")
+    ap.add_argument("--out", default="tuning_out")
     ap.add_argument("--seconds", type=float, default=0.3)
     a = ap.parse_args()
     for n in a.names:

@@ -29,7 +29,7 @@
  *   void Formura_Finalize();                    -- Templates.hs:383-386
  *       downloads a resident state into formura_data, frees device memory.
  *
- * Not exported (single GPU): Formura_Custom_Init(Formura_Navi *,
+ * Not exported (the library drives one GPU): Formura_Custom_Init(Formura_Navi *,
  * MPI_Comm) (Templates.hs:230-231) and the MPI Navi fields; a config with
  * mpi_shape is rejected by the generator.
  *
