@@ -55,6 +55,12 @@ PROGRAMS: Dict[str, tuple] = {
     "heat3d_64_tb_stream": ("heat3d.fmr", "heat3d_64_tb.yaml", {"kind": "stream3d", "ty": 4, "tz": 16, "cx": 16, "k": 3}),
     "heat3d_32_stream512": ("heat3d.fmr", "heat3d_32.yaml", {"kind": "stream3d", "ty": 16, "tz": 32, "cx": 8, "k": 2, "threads": 512}),
     "gs_64_flat": ("grayscott2d.fmr", "gs_64.yaml", "flat"),
+    # grids no power of two divides: 72 x 120 and 24 x 40 x 56
+    "gs_72x120": ("grayscott2d.fmr", "gs_72x120.yaml", "auto"),
+    "gs_72x120_tile": ("grayscott2d.fmr", "gs_72x120.yaml", {"kind": "tile2d", "tx": 24, "ty": 40, "k": 3}),
+    "gs_72x120_wave": ("grayscott2d.fmr", "gs_72x120.yaml", {"kind": "wave2d", "rows": 8, "k": 2}),
+    "heat3d_24x40x56": ("heat3d.fmr", "heat3d_24x40x56.yaml", "auto"),
+    "heat3d_24x40x56_stream": ("heat3d.fmr", "heat3d_24x40x56.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8, "k": 2}),
     "euler3d_16_inline": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1}),
     "euler3d_32_inline_stream": ("euler3d.fmr", "euler3d_32.yaml",
                                  {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 1, "inline": 1}),

@@ -1526,7 +1526,7 @@ def choose_schedule(em, want):
     return FlatSchedule()
 
 
-# --- driver --------------------------------------------------------------------------
+# --- entry points --------------------------------------------------------------------
 
 def generate(fmr_path: str, yaml_path: str, name: str, schedule: str = "auto"):
     """Front-end + emitters: returns (header text, hip source text, mm program)."""

@@ -38,6 +38,8 @@ CONFIGS = {
     "heat3d_64_tb": ("heat3d_ref.c", {"NX": 64, "TB": 1, "NT": 4, "GB1": 8, "GB2": 8, "GB3": 8}),
     "heat3d_256": ("heat3d_ref.c", {"NX": 256}),
     "gs_64": ("grayscott2d_ref.c", {"NX": 64}),
+    "gs_72x120": ("grayscott2d_ref.c", {"NX": 72, "NY": 120}),
+    "heat3d_24x40x56": ("heat3d_ref.c", {"NX": 24, "NY": 40, "NZ": 56}),
     "gs_2048": ("grayscott2d_ref.c", {"NX": 2048}),
     "euler3d_16": ("euler3d_ref.c", {"NX": 16}),
     "euler3d_32": ("euler3d_ref.c", {"NX": 32}),

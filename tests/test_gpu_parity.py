@@ -110,6 +110,23 @@ def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     _assert_same(o, h)
 
 
+@pytest.mark.parametrize("name,oracle,steps,resident", [
+    ("gs_72x120", "gs_72x120", 37, False),                  # default schedule (8 x 8 tiles)
+    ("gs_72x120", "gs_72x120", 37, True),
+    ("gs_72x120_tile", "gs_72x120", 41, True),              # 24 x 40 tiles, 3 steps per launch + remainder
+    ("gs_72x120_wave", "gs_72x120", 29, True),              # register windows on a 72-row grid
+    ("heat3d_24x40x56", "heat3d_24x40x56", 13, False),      # default (flat) schedule
+    ("heat3d_24x40x56_stream", "heat3d_24x40x56", 15, True),  # 8 x 8 columns along a 24-plane axis
+])
+def test_grids_no_power_of_two_divides(name, oracle, steps, resident):
+    """Unequal sides that no power of two above 8 divides (72 x 120, 24 x 40 x 56)."""
+    print("%s: %d steps, resident %s" % (name, steps, resident), flush=True)
+    seed = (lambda lib: inputs.heat_field(19, lib.grid)) if name.startswith("heat") else _gs_seed
+    o, h = _run_pair(name, oracle, seed, steps, resident=resident)
+    _assert_same(o, h)
+    assert o.navi_dict()["offset_x"] == (-steps) % o.grid[0]
+
+
 @pytest.mark.parametrize("name,oracle,steps", [("heat3d_256", "heat3d_256", 6)])
 def test_full_size_tuned_schedule_bit_exact(name, oracle, steps):
     o, h = _run_pair(name, oracle, lambda lib: inputs.heat_field(17, lib.grid), steps, resident=True)

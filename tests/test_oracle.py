@@ -87,6 +87,27 @@ def test_no_blocking_and_temporal_blocking_bit_identical(nb, tb, forwards):
     np.testing.assert_array_equal(a.field("q"), b.field("q"))
 
 
+@pytest.mark.parametrize("m", [(1, 0, 0), (0, 3, 0), (2, 5, 7)])
+def test_heat_fourier_mode_uneven_grid(m):
+    """The same closed form on a 24 x 40 x 56 grid (length 1 per axis, so
+    d_a = 1/N_a and dt = dx^2/8): lambda = 1 - 4 sum_a (dt/d_a^2) sin^2(pi m_a/N_a)
+    with dt/d_a^2 = (N_a/N_x)^2/8; an axis or extent mix-up fails it."""
+    o = _heat("heat3d_24x40x56")
+    n = o.grid
+    assert list(n) == [24, 40, 56]
+    i = [np.arange(x) for x in n]
+    q0 = np.cos(2 * np.pi * (m[0] * i[0][:, None, None] / n[0] + m[1] * i[1][None, :, None] / n[1]
+                             + m[2] * i[2][None, None, :] / n[2]))
+    o.set_state({"q": q0})
+    lam = 1 - 4 * sum((na / n[0]) ** 2 / 8 * math.sin(math.pi * ma / na) ** 2 for ma, na in zip(m, n))
+    for k in range(1, 4):
+        o.forward(1)
+        expect = lam ** k * np.roll(q0, shift=(k, k, k), axis=(0, 1, 2))
+        # per-axis factors up to 0.68 (q/dz^2 ~ 3e3 before the dt product): rounding ~1e-13
+        np.testing.assert_allclose(o.field("q"), expect, rtol=0, atol=1e-12)
+        assert (o.navi.offset_x, o.navi.offset_y, o.navi.offset_z) == ((-k) % 24, (-k) % 40, (-k) % 56)
+
+
 def _gs():
     o = oracle_lib("gs_64", layout_of("gs_64"))
     o.init()
@@ -142,6 +163,24 @@ def test_grayscott_linear_v_mode(m):
     o.forward(1)
     np.testing.assert_allclose(o.field("v"), e * g * np.roll(c, (1, 1), (0, 1)), rtol=0, atol=1e-16)
     np.testing.assert_allclose(o.field("u"), DT * F, rtol=1e-15)
+
+
+@pytest.mark.parametrize("m", [(1, 0), (0, 1), (5, 7)])
+def test_grayscott_linear_u_mode_uneven_grid(m):
+    """As above on a 72 x 120 grid (dx = 0.01 on both axes): the decay factor
+    uses each axis's own extent."""
+    o = oracle_lib("gs_72x120", layout_of("gs_72x120"))
+    o.init()
+    n = o.grid
+    assert list(n) == [72, 120]
+    i, j = np.arange(n[0]), np.arange(n[1])
+    c = np.cos(2 * np.pi * (m[0] * i[:, None] / n[0] + m[1] * j[None, :] / n[1]))
+    e = 1e-3
+    o.set_state({"u": 1 + e * c, "v": np.zeros(n)})
+    g = 1 - DT * (DU * 4 / DX ** 2 * sum(math.sin(math.pi * a / na) ** 2 for a, na in zip(m, n)) + F)
+    o.forward(4)
+    assert np.all(o.field("v") == 0)
+    np.testing.assert_allclose(o.field("u") - 1, e * g ** 4 * np.roll(c, (4, 4), (0, 1)), rtol=0, atol=1e-15)
 
 
 def test_grayscott_init_square():
