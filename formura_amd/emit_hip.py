@@ -264,8 +264,7 @@ class LibEmitter:
         L.append("constexpr int STEPS_PER_FORWARD = %d;" % self.cfg.steps_per_forward)
         L.append("constexpr size_t STATE_BYTES = sizeof(%s);" % tname)
         for i, (n, ct) in enumerate(self.fields):
-            L.append("#define FLD%d(st) ((%s%s*)((st) + offsetof(%s, %s)))"
-                     % (i, "const " if False else "", ct, tname, n))
+            L.append("#define FLD%d(st) ((%s*)((st) + offsetof(%s, %s)))" % (i, ct, tname, n))
         L.append("const char* const kDescription = R\"FMRJSON(%s)FMRJSON\";" % json.dumps(desc))
         plans = [p for p in (self.step, self.filter, self.first) if p is not None]
         for p in plans:
