@@ -40,6 +40,7 @@ CONFIGS = {
     "gs_64": ("grayscott2d_ref.c", {"NX": 64}),
     "gs_72x120": ("grayscott2d_ref.c", {"NX": 72, "NY": 120}),
     "heat3d_24x40x56": ("heat3d_ref.c", {"NX": 24, "NY": 40, "NZ": 56}),
+    "heat3d_24x40x56_tb": ("heat3d_ref.c", {"NX": 24, "NY": 40, "NZ": 56, "TB": 1, "NT": 2, "GB1": 7, "GB2": 11, "GB3": 6}),
     "gs_2048": ("grayscott2d_ref.c", {"NX": 2048}),
     "euler3d_16": ("euler3d_ref.c", {"NX": 16}),
     "euler3d_32": ("euler3d_ref.c", {"NX": 32}),

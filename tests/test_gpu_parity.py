@@ -117,6 +117,8 @@ def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     ("gs_72x120_wave", "gs_72x120", 29, True),              # register windows on a 72-row grid
     ("heat3d_24x40x56", "heat3d_24x40x56", 13, False),      # default (flat) schedule
     ("heat3d_24x40x56_stream", "heat3d_24x40x56", 15, True),  # 8 x 8 columns along a 24-plane axis
+    ("heat3d_24x40x56_tb", "heat3d_24x40x56_tb", 5, False),  # 2 steps per Forward, blocks 7 x 11 x 6
+    ("heat3d_24x40x56_tb_stream", "heat3d_24x40x56_tb", 6, True),
 ])
 def test_grids_no_power_of_two_divides(name, oracle, steps, resident):
     """Unequal sides that no power of two above 8 divides (72 x 120, 24 x 40 x 56)."""
@@ -124,7 +126,7 @@ def test_grids_no_power_of_two_divides(name, oracle, steps, resident):
     seed = (lambda lib: inputs.heat_field(19, lib.grid)) if name.startswith("heat") else _gs_seed
     o, h = _run_pair(name, oracle, seed, steps, resident=resident)
     _assert_same(o, h)
-    assert o.navi_dict()["offset_x"] == (-steps) % o.grid[0]
+    assert o.navi_dict()["offset_x"] == (-steps * o.layout["steps_per_forward"]) % o.grid[0]
 
 
 @pytest.mark.parametrize("name,oracle,steps", [("heat3d_256", "heat3d_256", 6)])

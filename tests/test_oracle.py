@@ -71,7 +71,8 @@ def test_heat_to_pos_follows_offset():
         assert o.to_pos("x", i) == o.navi.space_interval_x * ((i + (-3) % n) % n)
 
 
-@pytest.mark.parametrize("nb,tb,forwards", [("heat3d_16", "heat3d_16_tb", 3), ("heat3d_64", "heat3d_64_tb", 2)])
+@pytest.mark.parametrize("nb,tb,forwards", [("heat3d_16", "heat3d_16_tb", 3), ("heat3d_64", "heat3d_64_tb", 2),
+                                              ("heat3d_24x40x56", "heat3d_24x40x56_tb", 3)])
 def test_no_blocking_and_temporal_blocking_bit_identical(nb, tb, forwards):
     a = oracle_lib(nb, layout_of(nb))
     b = oracle_lib(tb, layout_of(tb))
