@@ -195,6 +195,41 @@ def run_fp32_single(steps: int = 10000, seed: int = 7):
     return out
 
 
+def run_fp32_diffusion(steps: int = 10000, seed: int = 7):
+    """2-D diffusion (diffusion2d.fmr: 5-point FTCS, dt = dx^2/10) on a
+    2048^2 periodic grid for 10^4 steps, as a one-instance batch in FP64 and in
+    the two FP32 forms (f32k: FP32 state and arithmetic, Kahan-compensated
+    accumulation; f32c: and the increment in float-float FP32): GLUPS, and the
+    error of each FP32 run against FP64 (relative L2, maximum absolute,
+    relative maximum).  Initial state: a Gaussian bump (sigma = 256 cells) plus
+    1 % seeded uniform noise."""
+    import numpy as np
+    from formura_amd import batch as B, inputs
+    n = 2048
+    i = np.arange(n)[:, None] - n / 2
+    j = np.arange(n)[None, :] - n / 2
+    q0 = np.exp(-(i * i + j * j) / (2 * (n / 8) ** 2)) + 0.01 * inputs.uniform(seed, (n, n))
+    out = {"grid": [n, n], "steps": steps, "program": "diffusion2d.fmr (dt = dx^2/10)"}
+    res = {}
+    for name in ("diff_batch_f64", "diff_batch_f32k", "diff_batch_f32c"):
+        e = B.Ensemble(name, [(n, n)], np.zeros((1, 1)))
+        e.set_state(0, {"q": q0})
+        e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"])  # warm-up, captures the graph
+        e.set_state(0, {"q": q0})
+        ms = e.step(steps)
+        res[name] = e.get_state(0)["q"]
+        out[name] = {"glups": n * n * steps / (ms * 1e-3) / 1e9, "ms": ms}
+        e.close()
+    x = res["diff_batch_f64"]
+    for prec in ("f32k", "f32c"):
+        y = res["diff_batch_" + prec]
+        out[prec + "_error_vs_fp64"] = {"q": {
+            "rel_l2": float(np.sqrt(np.sum((y - x) ** 2) / np.sum(x ** 2))),
+            "max_abs": float(np.max(np.abs(y - x))),
+            "rel_max": float(np.max(np.abs(y - x)) / np.max(np.abs(x)))}}
+    return out
+
+
 def run_per_instance_fp64(sizes, steps: int, probe_steps: int = 600):
     """The plain per-instance FP64 GPU path for the same ensemble: every instance
     advanced by its own plain Formura_Forward library (state resident, one
@@ -523,6 +558,7 @@ def main():
     if "plain" not in legs:
         line = {"legs": sorted(legs)}
         for key, leg, fn in (("grayscott2048_fp32", "fp32", run_fp32_single),
+                             ("diffusion2d_2048_fp32", "fp32", run_fp32_diffusion),
                              ("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
                              ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples),
                              ("adjoint_probe", "adjoint_probe", run_adjoint_probe)):
@@ -572,6 +608,7 @@ def main():
         except Exception as e:
             line["ensemble"] = {"error": str(e)[:300]}
     for key, leg, fn in (("grayscott2048_fp32", "fp32", run_fp32_single),
+                         ("diffusion2d_2048_fp32", "fp32", run_fp32_diffusion),
                          ("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
                          ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples)):
         if leg in legs:
