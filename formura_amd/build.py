@@ -62,6 +62,8 @@ PROGRAMS: Dict[str, tuple] = {
     "heat3d_24x40x56": ("heat3d.fmr", "heat3d_24x40x56.yaml", "auto"),
     "heat3d_24x40x56_stream": ("heat3d.fmr", "heat3d_24x40x56.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8, "k": 2}),
     "heat3d_24x40x56_tb": ("heat3d.fmr", "heat3d_24x40x56_tb.yaml", "auto"),
+    "diffusion2d_64": ("diffusion2d.fmr", "diffusion2d_64.yaml", "auto"),
+    "diffusion2d_64_wave": ("diffusion2d.fmr", "diffusion2d_64.yaml", {"kind": "wave2d", "rows": 8, "k": 3}),
     "heat3d_24x40x56_tb_stream": ("heat3d.fmr", "heat3d_24x40x56_tb.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8, "k": 2}),
     "euler3d_16_inline": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1}),
     "euler3d_32_inline_stream": ("euler3d.fmr", "euler3d_32.yaml",

@@ -187,3 +187,31 @@ def test_diffusion_batch_f64_exact_and_fp32_errors():
                 for a, b in zip(res["diff_batch_f64"], res["diff_batch_" + prec])]
         print(prec, "relative L2 vs f64 after 2000 steps:", ["%.2e" % x for x in errs], flush=True)
         assert max(errs) < 1e-6, (prec, errs)
+
+
+@pytest.mark.parametrize("name", ["diffusion2d_64", "diffusion2d_64_wave"])
+def test_diffusion_plain_path_matches_batch_and_frontend(name):
+    """The plain Formura_Forward library of the same diffusion program (default
+    tile2d and register-window schedules) gives the f64 batch's bits and the
+    front-end graph's, and runs the init graph (q = 1 everywhere)."""
+    import mm_interp
+    h = hip_lib(name)
+    h.init()
+    assert np.all(h.field("q") == 1.0)
+    q0 = _smooth_noisy((64, 64), 21)
+    h.set_state({"q": q0})
+    e = _ens("diff_batch_f64", [(64, 64)], np.zeros((1, 1)))
+    e.set_state(0, {"q": q0})
+    h.set_resident(True)
+    h.upload()
+    h.forward_steps(45)
+    h.download()
+    h.set_resident(False)
+    e.step(45)
+    cpu = {"q": q0.copy()}
+    for _ in range(45):
+        cpu = mm_interp.step(_diffusion_mm().step, 1, cpu)
+    np.testing.assert_array_equal(h.field("q"), cpu["q"])
+    np.testing.assert_array_equal(e.get_state(0)["q"], cpu["q"])
+    assert e.offsets(0) == [h.navi.offset_x, h.navi.offset_y] == [(-45) % 64, (-45) % 64]
+    e.close()
