@@ -270,6 +270,44 @@ def run_heat_roofline(name: str = "heat3d_512", steps: int = 200):
             "mean_launch_us": r["mean_launch_us"], "steps": r["steps"]}
 
 
+def run_fp32_heat(steps: int = 200, seed: int = 23):
+    """3-D heat on 512^3 from one seeded state (i.i.d. U[0,1)): the plain FP64
+    library (heat3d_512) against heat3d_kahan.fmr, the same 7-point step on FP32
+    state with Kahan-compensated accumulation written in the language (fields
+    q and c, FP32 arithmetic), both on the stream3d schedule, state resident:
+    GLUPS and the FP32 error against FP64 (relative L2, maximum absolute), for
+    q and for the compensated value q - c."""
+    import numpy as np
+    from formura_amd import inputs
+    from formura_amd.runtime import open_program
+    out = {"steps": steps, "grid": [512, 512, 512]}
+    res = {}
+    for name in ("heat3d_512", "heat3d_kahan_512"):
+        lib = open_program(name)
+        lib.init()
+        q0 = inputs.uniform(seed, lib.grid)
+        st = {"q": q0} if name == "heat3d_512" else {"q": q0.astype(np.float32),
+                                                      "c": np.zeros(lib.grid, np.float32)}
+        lib.set_state(st)
+        lib.set_resident(True)
+        lib.upload()
+        lib.forward_steps(20)  # warm-up
+        lib.set_state(st)
+        lib.upload()
+        ms = lib.forward_steps(steps)
+        lib.download()
+        res[name] = {f: lib.field(f).copy() for f in lib.fields}
+        out[name] = {"glups": float(np.prod(lib.grid)) * steps / (ms * 1e-3) / 1e9, "ms": ms,
+                     "bytes_per_lup": 16 if name == "heat3d_512" else 16}
+        lib.finalize()
+    x = res["heat3d_512"]["q"]
+    k = res["heat3d_kahan_512"]
+    for key, y in (("q", k["q"].astype(np.float64)), ("q_minus_c", k["q"].astype(np.float64) - k["c"])):
+        out["error_vs_fp64_" + key] = {"rel_l2": float(np.sqrt(np.sum((y - x) ** 2) / np.sum(x ** 2))),
+                                       "max_abs": float(np.max(np.abs(y - x)))}
+    return out
+
+
 def run_euler(steps: int = 200):
     """3-D Euler HLLC at 256^3: the plain FP64 path and two mixed forms (FP32 flux
     arithmetic; FP64 wave speeds, positivity checks, update and totals) from the
@@ -559,6 +597,7 @@ def main():
         line = {"legs": sorted(legs)}
         for key, leg, fn in (("grayscott2048_fp32", "fp32", run_fp32_single),
                              ("diffusion2d_2048_fp32", "fp32", run_fp32_diffusion),
+                             ("heat3d_512_fp32", "fp32", run_fp32_heat),
                              ("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
                              ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples),
                              ("adjoint_probe", "adjoint_probe", run_adjoint_probe)):
@@ -609,6 +648,7 @@ def main():
             line["ensemble"] = {"error": str(e)[:300]}
     for key, leg, fn in (("grayscott2048_fp32", "fp32", run_fp32_single),
                          ("diffusion2d_2048_fp32", "fp32", run_fp32_diffusion),
+                         ("heat3d_512_fp32", "fp32", run_fp32_heat),
                          ("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
                          ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples)):
         if leg in legs:

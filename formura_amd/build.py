@@ -63,6 +63,10 @@ PROGRAMS: Dict[str, tuple] = {
     "heat3d_24x40x56_stream": ("heat3d.fmr", "heat3d_24x40x56.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8, "k": 2}),
     "heat3d_24x40x56_tb": ("heat3d.fmr", "heat3d_24x40x56_tb.yaml", "auto"),
     "diffusion2d_64": ("diffusion2d.fmr", "diffusion2d_64.yaml", "auto"),
+    # FP32 state with Kahan-compensated accumulation written in the language (fields q, c)
+    "heat3d_kahan_16": ("heat3d_kahan.fmr", "heat3d_kahan_16.yaml", "flat"),
+    "heat3d_kahan_16_stream": ("heat3d_kahan.fmr", "heat3d_kahan_16.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8, "k": 2}),
+    "heat3d_kahan_512": ("heat3d_kahan.fmr", "heat3d_kahan_512.yaml", {"kind": "stream3d", "ty": 16, "tz": 64, "cx": 64, "k": 2, "threads": 512}),
     "diffusion2d_64_wave": ("diffusion2d.fmr", "diffusion2d_64.yaml", {"kind": "wave2d", "rows": 8, "k": 3}),
     "heat3d_24x40x56_tb_stream": ("heat3d.fmr", "heat3d_24x40x56_tb.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8, "k": 2}),
     "euler3d_16_inline": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1}),

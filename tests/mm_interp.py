@@ -259,7 +259,8 @@ def step_typed(mm_graph, sleeve, fields, offsets=None):
                 a, b = vals[m.inst[2]], vals[m.inst[3]]
                 wide = np.float64 if np.float64 in (types[m.inst[2]], types[m.inst[3]]) else np.float32
                 a, b = np.asarray(a, dtype=wide), np.asarray(b, dtype=wide)
-                v = {"+": a + b, "-": a - b, "*": a * b, "/": a / b}[m.inst[1]]
+                op = m.inst[1]
+                v = a + b if op == "+" else a - b if op == "-" else a * b if op == "*" else a / b
             elif kk == "Store":
                 out[m.inst[1]] = np.array(np.broadcast_to(vals[m.inst[2]], shape)).astype(fields[m.inst[1]].dtype)
                 vals.append(None)
