@@ -297,8 +297,8 @@ def run_fp32_heat(steps: int = 200, seed: int = 23):
         ms = lib.forward_steps(steps)
         lib.download()
         res[name] = {f: lib.field(f).copy() for f in lib.fields}
-        out[name] = {"glups": float(np.prod(lib.grid)) * steps / (ms * 1e-3) / 1e9, "ms": ms,
-                     "bytes_per_lup": 16 if name == "heat3d_512" else 16}
+        # 16 B per cell update in both: one double, or the two floats q and c, read and written
+        out[name] = {"glups": float(np.prod(lib.grid)) * steps / (ms * 1e-3) / 1e9, "ms": ms, "bytes_per_lup": 16}
         lib.finalize()
     x = res["heat3d_512"]["q"]
     k = res["heat3d_kahan_512"]
