@@ -74,7 +74,7 @@ def _seed_state(lib, seed):
     if key not in _STATE_CACHE:
         if "u" in lib.fields and "v" in lib.fields:
             st = inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), seed)
-        elif set(lib.fields) == {"r", "mx", "my", "mz", "e"}:
+        elif set(lib.fields) == {"rho", "rhou", "rhov", "rhow", "e"}:
             st = inputs.euler_state(1, lib.grid)
         else:
             st = {f: inputs.uniform(seed + i, lib.grid) for i, f in enumerate(lib.fields)}

@@ -72,8 +72,10 @@ PROGRAMS: Dict[str, tuple] = {
     "euler3d_16_inline": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1}),
     "euler3d_32_inline_stream": ("euler3d.fmr", "euler3d_32.yaml",
                                  {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 1, "inline": 1}),
-    "euler3d_16_staged": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "staged3d", "ty": 8, "tz": 16, "cx": 4}),
-    "euler3d_32_staged": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "staged3d", "ty": 16, "tz": 16, "cx": 8, "threads": 512}),
+    "euler3d_16_staged": ("euler3d.fmr", "euler3d_16.yaml",
+                          {"kind": "staged3d", "ty": 8, "tz": 16, "cx": 4, "inline": "inner"}),
+    "euler3d_32_staged": ("euler3d.fmr", "euler3d_32.yaml",
+                          {"kind": "staged3d", "ty": 16, "tz": 16, "cx": 8, "threads": 512, "inline": "inner"}),
     "index2d_32_tb": ("index2d.fmr", "index2d_32_tb.yaml", {"kind": "tile2d", "tx": 16, "ty": 16, "k": 2}),
     "index2d_32_tb8": ("index2d.fmr", "index2d_32_tb.yaml", {"kind": "tile2d", "tx": 8, "ty": 32, "k": 2, "threads": 128}),
     "index3d_16_tb": ("index3d.fmr", "index3d_16_tb.yaml", {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 2}),
@@ -87,12 +89,14 @@ PROGRAMS: Dict[str, tuple] = {
     "euler3d_16_mixed": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),
     "euler3d_32_mixed": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),
     "euler3d_32_mixed_mat": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "flat", "mixed": 1}),
+    "euler3d_32_mixed_inner": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "flat", "mixed": 1, "inline": "inner"}),
     "euler3d_32_mixed_staged": ("euler3d.fmr", "euler3d_32.yaml",
-                                {"kind": "staged3d", "ty": 8, "tz": 16, "cx": 8, "mixed": 1}),
+                                {"kind": "staged3d", "ty": 8, "tz": 16, "cx": 8, "mixed": 1, "inline": "inner"}),
     "euler3d_32_mixed_stream": ("euler3d.fmr", "euler3d_32.yaml",
                                 {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 1, "inline": 1, "mixed": 1}),
     "euler3d_256_mixed_staged": ("euler3d.fmr", "euler3d_256.yaml",
-                                 {"kind": "staged3d", "ty": 4, "tz": 64, "cx": 32, "threads": 512, "mixed": 1}),
+                                 {"kind": "staged3d", "ty": 4, "tz": 64, "cx": 32, "threads": 512, "mixed": 1,
+                                  "inline": "inner"}),
     "euler3d_256_mixed": ("euler3d.fmr", "euler3d_256.yaml",
                           {"kind": "stream3d", "ty": 8, "tz": 32, "cx": 32, "k": 1, "inline": 1, "mixed": 1}),
 }

@@ -56,6 +56,12 @@ class LibEmitter:
             from .emit_hip import _tuned_spec
             spec = _tuned_spec(name)
         self.inlined = bool(isinstance(spec, dict) and spec.get("inline"))
+        # "inline": "inner" recomputes only the temporaries no Store node reads
+        # (wave speeds read by the flux temporaries) inside their readers;
+        # the temporaries the update reads stay materialized
+        self.inline_inner = self.inlined and spec.get("inline") == "inner"
+        if self.inline_inner:
+            self.inlined = False
         # mixed precision: FP32 arithmetic except branch decisions, square-root
         # arguments and the update chain (emit.mixed_precision), decided per
         # node before any inlining
@@ -66,7 +72,21 @@ class LibEmitter:
             # temporaries stay materialized: precision decided over the whole graph
             self._prec, storage = mixed_precision_graph(mm.step)
         step_mm = mm.step
-        if self.inlined:
+        if self.inline_inner:
+            from .manifest import inline_temporaries, inner_temporaries
+            inner = inner_temporaries(mm.step)
+            keep = frozenset(k for k in mm.step if mm.step[k].micro[-1].inst[0] != "Store" and k not in inner)
+            orig_rng = calc_range(mm.step)
+            step_mm = inline_temporaries(mm.step, keep=keep)
+            new_rng = calc_range(step_mm)
+            for k in step_mm:
+                if (orig_rng[k].lower, orig_rng[k].upper) != (new_rng[k].lower, new_rng[k].upper):
+                    raise EmitError("inlining changed the stencil range of %s" % step_mm[k].name)
+            if self.mixed:
+                # the precision of the remaining temporaries is decided over the
+                # graph the kernels evaluate
+                self._prec, storage = mixed_precision_graph(step_mm)
+        elif self.inlined:
             from .manifest import inline_temporaries
             orig_rng = calc_range(mm.step)
             step_mm = inline_temporaries(mm.step, prec=self._prec)
@@ -1436,13 +1456,18 @@ def candidate_specs(em) -> List[dict]:
                         out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": 1, "inline": 1,
                                     "threads": 512})
             # ... or evaluating each of them once per point into LDS rings
+            # (temporaries that only other temporaries read, such as the wave
+            # speeds a flux reads, recomputed inside their readers)
+            from .manifest import inner_temporaries
+            extra = {"inline": "inner"} if inner_temporaries(em.mm.step) else {}
+            em_s = LibEmitter(em.mm, em.name, dict({"kind": "flat"}, **extra)) if extra else em
             for ty, tz in ((8, 16), (16, 16), (8, 32), (16, 32), (4, 64), (8, 64)):
                 for cx in (8, 16, 32):
-                    if (_staged3d_ok(em, ty, tz, cx)
+                    if (_staged3d_ok(em_s, ty, tz, cx)
                             and (em.N[0] // cx) * (em.N[1] // ty) * (em.N[2] // tz) >= 256):
                         for nth in (256, 512):
                             if nth <= 2 * ty * tz:
-                                spec = {"kind": "staged3d", "ty": ty, "tz": tz, "cx": cx}
+                                spec = dict({"kind": "staged3d", "ty": ty, "tz": tz, "cx": cx}, **extra)
                                 if nth != 256:
                                     spec["threads"] = nth
                                 out.append(spec)

@@ -68,10 +68,11 @@ def euler_primitive(seed: int, shape):
 
 
 def euler_state(seed: int, shape) -> dict:
-    """Conserved fields r, mx, my, mz, e of ``euler_primitive``."""
+    """Conserved fields rho, rhou, rhov, rhow (momentum), e (total energy) of
+    ``euler_primitive``: the field names of the Euler program."""
     rho, u, v, w, p = euler_primitive(seed, shape)
     e = p / (EULER_GAMMA - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
-    return {"r": rho, "mx": rho * u, "my": rho * v, "mz": rho * w, "e": e}
+    return {"rho": rho, "rhou": rho * u, "rhov": rho * v, "rhow": rho * w, "e": e}
 
 
 def euler_cfl_dt(seed: int, shape, dx: float, cfl: float = 0.4) -> float:

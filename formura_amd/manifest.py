@@ -174,13 +174,29 @@ def gen_mm_program(om: OMProgram) -> MMProgram:
     )
 
 
-def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = None) -> Dict[int, MMNode]:
+def inner_temporaries(mm: Dict[int, MMNode]) -> set:
+    """The manifest temporaries that no Store node reads (the wave speeds a
+    flux temporary reads, for instance): ``inline_temporaries(mm, keep=...)``
+    with the complement keeps only the temporaries the update reads."""
+    read_by_store = set()
+    for node in mm.values():
+        if node.micro[-1].inst[0] == "Store":
+            read_by_store |= {m.inst[2] for m in node.micro if m.inst[0] == "LoadCursor"}
+    return {k for k, n in mm.items() if n.micro[-1].inst[0] != "Store" and k not in read_by_store}
+
+
+def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = None,
+                       keep=frozenset()) -> Dict[int, MMNode]:
     """Replace every ``LoadCursor(c, X)`` of the Store nodes by X's own micro DAG
     with its cursors shifted by ``c`` (recursively), sharing identical
     instructions.  A manifest temporary holds, at each input position, the
     value of its DAG at that position (Templates.hs:442-474 index it so), so
     recomputing it where it is read gives the same numbers; the Store nodes of
     the result read only the state fields.  Returns the Store nodes only.
+
+    ``keep``: temporaries that stay materialized.  They are returned too, reads
+    of them stay ``LoadCursor`` (their cursor shifted like any other), and the
+    other temporaries are inlined into them as into the Store nodes.
 
     ``prec`` (node -> micro -> C type, from emit.mixed_precision on the nodes
     before inlining) is carried into the copies and is part of the sharing key,
@@ -191,7 +207,7 @@ def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = 
     out = {}
     for k in sorted(mm):
         node = mm[k]
-        if node.micro[-1].inst[0] != "Store":
+        if node.micro[-1].inst[0] != "Store" and k not in keep:
             continue
         new: List[Micro] = []
         memo: Dict[tuple, int] = {}
@@ -212,6 +228,8 @@ def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = 
             emit_ = lambda inst, typ: emit(inst, typ, p)
             if k0 == "LoadCursorStatic":
                 r = emit_(("LoadCursorStatic", _add(m.inst[1], shift), m.inst[2]), m.typ)
+            elif k0 == "LoadCursor" and m.inst[2] in keep:
+                r = emit_(("LoadCursor", _add(m.inst[1], shift), m.inst[2]), m.typ)
             elif k0 == "LoadCursor":
                 tgt = mm[m.inst[2]]
                 r = copy(tgt, _add(m.inst[1], shift), len(tgt.micro) - 1, {})
@@ -244,6 +262,8 @@ def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = 
             if m.inst[0] in ("LoadCursorStatic", "LoadCursor"):
                 dim = len(m.inst[1])
                 break
-        copy(node, (0,) * (dim or 3), len(node.micro) - 1, {})
+        root = copy(node, (0,) * (dim or 3), len(node.micro) - 1, {})
+        if root != len(new) - 1:  # the node's value must be its last instruction
+            new.append(dataclasses.replace(new[root]))
         out[k] = MMNode(k, new, node.typ, node.name)
     return out

@@ -45,6 +45,15 @@ CONFIGS = {
     "euler3d_16": ("euler3d_ref.c", {"NX": 16}),
     "euler3d_32": ("euler3d_ref.c", {"NX": 32}),
     "euler3d_256": ("euler3d_ref.c", {"NX": 256}),
+    # the reference's shock tube (sample/3d_hllc: 400 cells over 4.0 along the
+    # tube, dx = 0.01, dt = C_cfl*dx/max_signal_speed = (0.8*0.01)/4, :19-21)
+    # along each axis in turn, 4 cells across
+    "euler3d_tube_x": ("euler3d_ref.c", dict(NX=400, NY=4, NZ=4, LX=4.0, LY=0.04, LZ=0.04,
+                                               DX=0.01, DY=0.01, DZ=0.01, DT="((0.8*0.01)/4.0)")),
+    "euler3d_tube_y": ("euler3d_ref.c", dict(NX=4, NY=400, NZ=4, LX=0.04, LY=4.0, LZ=0.04,
+                                               DX=0.01, DY=0.01, DZ=0.01, DT="((0.8*0.01)/4.0)")),
+    "euler3d_tube_z": ("euler3d_ref.c", dict(NX=4, NY=4, NZ=400, LX=0.04, LY=0.04, LZ=4.0,
+                                               DX=0.01, DY=0.01, DZ=0.01, DT="((0.8*0.01)/4.0)")),
 }
 
 # static arrays beyond 2 GiB need the medium code model
@@ -55,17 +64,27 @@ def lib_path(name: str, kind: str = "strict") -> str:
     return os.path.join(OUT, "liboracle_%s_%s.so" % (name, kind))
 
 
-def build(name: str, kind: str = "strict", force: bool = False) -> str:
+def build(name: str, kind: str = "strict", force: bool = False, source_text: str = None,
+          out: str = None) -> str:
+    """Compile configuration ``name``.  ``source_text`` replaces the source
+    file's text (tests plant mistakes in a copy this way) and ``out`` names
+    the library it goes to."""
     src, defs = CONFIGS[name]
     flags = (STRICT if kind == "strict" else FAST) + ["-D%s=%s" % kv for kv in sorted(defs.items())]
     if name in LARGE:
         flags.append("-mcmodel=medium")
     srcp = os.path.join(HERE, src)
-    out = lib_path(name, kind)
-    stamp = hashlib.sha1((" ".join(flags) + open(srcp).read()).encode()).hexdigest()
+    text = open(srcp).read() if source_text is None else source_text
+    out = out or lib_path(name, kind)
+    if source_text is not None:
+        srcp = out + ".c"
+        os.makedirs(os.path.dirname(srcp), exist_ok=True)
+        with open(srcp, "w") as f:
+            f.write(text)
+    stamp = hashlib.sha1((" ".join(flags) + text).encode()).hexdigest()
     if not force and os.path.exists(out) and os.path.exists(out + ".stamp") and open(out + ".stamp").read() == stamp:
         return out
-    os.makedirs(OUT, exist_ok=True)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
     cmd = ["gcc"] + flags + ["-o", out + ".tmp", srcp, "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -163,12 +163,16 @@ def test_reference_programs_compile(fmr, yml, sleeve, manifest):
 
 
 def test_euler_frontend_graph_matches_oracle_bit_exact():
-    """The HLLC program: 15 manifest flux temporaries + 5 stores, through the
-    numpy interpreter of the front-end's graphs, against the oracle."""
+    """The HLLC program: 24 manifest temporaries (S_L, S_R, S_M and five flux
+    components per face direction) + 5 stores, through the numpy interpreter
+    of the front-end's graphs, against the oracle."""
     name = "euler3d_16"
     mm = _mm(name)
     assert len([n for n in mm.step.values() if n.name.endswith("_next")]) == 5
-    assert len(mm.step) == 20
+    assert len(mm.step) == 29
+    rng = calc_range(mm.step)
+    assert all((r.lower, r.upper) == ((0, 1) if not mm.step[k].name.endswith("_next") else (-1, 1))
+               for k, r in rng.items())
     o = oracle_lib(name, layout_of(name))
     o.init()
     st = inputs.euler_state(2, o.grid)
@@ -179,6 +183,43 @@ def test_euler_frontend_graph_matches_oracle_bit_exact():
     o.forward(3)
     for f in fields:
         np.testing.assert_array_equal(fields[f], o.field(f))
+
+
+FIRST_ORDER_LINE = "  #(U_xL, U_xR, U_yL, U_yR, U_zL, U_zR) = first_order_LR(U)   # first-order scheme\n"
+MUSCL_LINE = "  (U_xL, U_xR, U_yL, U_yR, U_zL, U_zR) = MUSCL_Hancock(U)   # second-order scheme\n"
+DT_LINE = "double :: dt = C_cfl*min3(dx,dy,dz)/max_signal_speed\n"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference checkout not present")
+def test_euler_program_is_the_reference_first_order_hllc(tmp_path):
+    """The bundled Euler program computes what the reference's own
+    sample/3d_hllc/3d_hllc.fmr computes with its first-order reconstruction
+    (first_order_LR in place of MUSCL_Hancock, :210-211) and the same fixed dt:
+    the same 24 temporaries with the same ranges and instruction counts, and
+    the same numbers bit for bit through the interpreter."""
+    text = open(os.path.join(REF, "sample/3d_hllc/3d_hllc.fmr")).read()
+    assert FIRST_ORDER_LINE in text and MUSCL_LINE in text and DT_LINE in text
+    ours = open(os.path.join(B.PROG_DIR, "euler3d.fmr")).read()
+    dt = [ln for ln in ours.splitlines() if ln.startswith("double :: dt")][0]
+    text = (text.replace(FIRST_ORDER_LINE, FIRST_ORDER_LINE.replace("#(", "(", 1))
+            .replace(MUSCL_LINE, "  #" + MUSCL_LINE[2:]).replace(DT_LINE, dt + "\n"))
+    p = tmp_path / "first_order.fmr"
+    p.write_text(text)
+    yml = os.path.join(B.PROG_DIR, "euler3d_16.yaml")
+    ref = generate(str(p), yml, "ref_first_order", "flat")[2]
+    mm = _mm("euler3d_16")
+    assert list(ref.om.state_signature) == list(mm.om.state_signature)
+    r_ref, r_mm = calc_range(ref.step), calc_range(mm.step)
+    shape = lambda g, r: [(r[k].lower, r[k].upper, len(g[k].micro)) for k in sorted(g)]
+    assert shape(ref.step, r_ref) == shape(mm.step, r_mm)
+    st = inputs.euler_state(4, mm.om.config.grid_per_node)
+    a = {k: v.copy() for k, v in st.items()}
+    b = {k: v.copy() for k, v in st.items()}
+    for _ in range(3):
+        a = mm_interp.step(ref.step, ref.om.config.sleeve, a)
+        b = mm_interp.step(mm.step, mm.om.config.sleeve, b)
+    for f in a:
+        np.testing.assert_array_equal(a[f], b[f], err_msg=f)
 
 
 def test_inlined_temporaries_match_oracle_bit_exact():

@@ -1,7 +1,8 @@
 """The ``staged3d`` schedule's plan, checked on the CPU.
 
 For a 3-D program whose manifest temporaries read only the state (the Euler
-HLLC program's face fluxes), ``Staged3DSchedule.analyse`` groups the
+HLLC program's face fluxes, once the wave speeds they read are recomputed
+inside them: ``"inline": "inner"``), ``Staged3DSchedule.analyse`` groups the
 temporaries by the offsets at which the update reads them and derives the LDS
 ring depths, halos and warm-up planes.  The kernel then addresses its rings
 with ``slot = (plane - xs + 64*D) mod D``.  The test replays that bookkeeping
@@ -20,10 +21,26 @@ from formura_amd import build as B
 from formura_amd.emit_hip import LibEmitter, Staged3DSchedule, candidate_specs, generate, EmitError
 
 
+INNER = {"kind": "flat", "inline": "inner"}
+
+
 def emitter(name="euler3d_32", spec=None):
     fmr, yml, _ = B.PROGRAMS[name]
     _, _, mm = generate(os.path.join(B.PROG_DIR, fmr), os.path.join(B.PROG_DIR, yml), name, "flat")
-    return LibEmitter(mm, name, spec or "flat"), mm
+    return LibEmitter(mm, name, spec or (INNER if name.startswith("euler") else "flat")), mm
+
+
+def test_wave_speeds_are_inner_temporaries():
+    """S_L, S_R, S_M of each direction are read only by the fluxes: inlined
+    into them, the fluxes read only the state, with their ranges unchanged."""
+    from formura_amd.manifest import inner_temporaries
+    em, mm = emitter()
+    inner = inner_temporaries(mm.step)
+    assert sorted(mm.step[k].name for k in inner) == sorted(["s_l", "s_r", "s_m"] * 3)
+    assert len(em.step.temps) == 15
+    assert all(m.inst[0] != "LoadCursor" for t in em.step.temps for m in em.step.mm[t].micro)
+    with pytest.raises(EmitError):  # without the inlining the fluxes read other temporaries
+        Staged3DSchedule.analyse(LibEmitter(mm, "euler3d_32", "flat"), 16, 16)
 
 
 def test_euler_groups_and_rings():
@@ -103,9 +120,10 @@ def test_ring_bookkeeping_catches_a_short_warm_up():
 
 def test_applicability():
     em, _ = emitter()
-    specs = [c for c in candidate_specs(emitter("euler3d_256")[0]) if c["kind"] == "staged3d"]
-    assert specs and all(Staged3DSchedule.lds_bytes(emitter("euler3d_256")[0], c["ty"], c["tz"]) <= 160 * 1024
-                         for c in specs)
+    specs = [c for c in candidate_specs(emitter("euler3d_256", "flat")[0]) if c["kind"] == "staged3d"]
+    assert specs and all(c.get("inline") == "inner" for c in specs)
+    assert all(Staged3DSchedule.lds_bytes(emitter("euler3d_256")[0], c["ty"], c["tz"]) <= 160 * 1024
+               for c in specs)
     heat, _ = emitter("heat3d_32")
     with pytest.raises(EmitError):
         LibEmitter(heat.mm, "heat3d_32", {"kind": "staged3d", "ty": 8, "tz": 8, "cx": 8})

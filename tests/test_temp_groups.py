@@ -32,9 +32,11 @@ def _check(plan):
     return groups
 
 
-def test_euler_fluxes_in_one_kernel():
+def test_euler_temporaries_in_three_levels():
+    """S_L, S_R (read the state), S_M (reads S_L, S_R), then the 15 flux
+    components (read all three): one kernel per level."""
     plan = _plan(os.path.join(B.PROG_DIR, "euler3d.fmr"), os.path.join(B.PROG_DIR, "euler3d_16.yaml"), "e")
-    assert [len(g) for g in _check(plan)] == [15]
+    assert [len(g) for g in _check(plan)] == [6, 3, 15]
 
 
 @pytest.mark.skipif(not B.reference_present(), reason="formura checkout not present")
