@@ -284,13 +284,20 @@ def generate_program(name: str, fmr: str, yaml: str, schedule: str = "auto", out
     return d
 
 
-def compile_library(name: str, src_dir: str, force: bool = False) -> str:
+def source_stamp(name: str, src_dir: str) -> str:
+    """Hash of everything a library is compiled from (its generated source and
+    header, the runtime and public headers, the compiler flags)."""
     src = os.path.join(src_dir, "%s.hip" % name)
-    out = os.path.join(src_dir, "lib%s.so" % name)
     deps = [src, os.path.join(src_dir, "%s.h" % name), os.path.join(CSRC, "formura_hip_rt.h"),
             os.path.join(INCLUDE, "formura_hip.h"), os.path.join(INCLUDE, "formura_batch.h"),
             os.path.join(INCLUDE, "formura_adjoint.h")]
-    stamp = hashlib.sha1(("\0".join(HIP_FLAGS) + "".join(open(p).read() for p in deps)).encode()).hexdigest()
+    return hashlib.sha1(("\0".join(HIP_FLAGS) + "".join(open(p).read() for p in deps)).encode()).hexdigest()
+
+
+def compile_library(name: str, src_dir: str, force: bool = False) -> str:
+    src = os.path.join(src_dir, "%s.hip" % name)
+    out = os.path.join(src_dir, "lib%s.so" % name)
+    stamp = source_stamp(name, src_dir)
     stamp_file = out + ".stamp"
     if not force and os.path.exists(out) and os.path.exists(stamp_file) and open(stamp_file).read() == stamp:
         return out
@@ -310,25 +317,89 @@ def build_one(name: str, force: bool = False) -> str:
     return compile_library(name, d, force=force)
 
 
+def regenerate(name: str) -> str:
+    """Write the current generated source of library ``name`` (whatever its
+    kind) into its build folder, compiling nothing; returns the folder."""
+    if name in BATCHES or name in ADJOINTS:
+        from .emit_hip import generate
+        if name in BATCHES:
+            from .emit_batch import BatchEmitter
+            fmr, yml, prec, params, tx, ty, k = BATCHES[name]
+            spec = _tuned_batch(name)
+            grid, threads = 2048, 256
+            if spec:
+                tx, ty, k = int(spec["tx"]), int(spec["ty"]), int(spec["k"])
+                grid = int(spec.get("grid", grid))
+                threads = int(spec.get("threads", threads))
+            _, _, mm = generate(os.path.join(PROG_DIR, fmr), os.path.join(PROG_DIR, yml), name, "flat")
+            src = BatchEmitter(mm, name, prec, params, tx, ty, k, grid_wgs=grid, threads=threads).emit()
+        else:
+            from .emit_adjoint import AdjointEmitter
+            fmr, yml, params, real = ADJOINTS[name]
+            _, _, mm = generate(os.path.join(PROG_DIR, fmr), os.path.join(PROG_DIR, yml), name, "flat")
+            src = AdjointEmitter(mm, name, params, real).emit()
+        d = os.path.join(BUILD_DIR, name)
+        p = os.path.join(d, "%s.hip" % name)
+        if not os.path.exists(p) or open(p).read() != src:
+            os.makedirs(d, exist_ok=True)
+            with open(p, "w") as f:
+                f.write(src)
+        return d
+    if name.startswith("gs_side_"):
+        d = os.path.join(BUILD_DIR, name)
+        return generate_program(name, os.path.join(PROG_DIR, "grayscott2d.fmr"), os.path.join(d, "%s.yaml" % name),
+                                dict(SIDE_SPEC), out_dir=d)
+    if name in SAMPLES:
+        rel, yml, schedule = SAMPLES[name]
+        return generate_program(name, os.path.join(REFERENCE_ROOT, rel), os.path.join(PROG_DIR, yml), schedule)
+    fmr, yaml, schedule = PROGRAMS[name]
+    return generate_program(name, os.path.join(PROG_DIR, fmr), os.path.join(PROG_DIR, yaml), schedule)
+
+
+def check_current(name: str) -> str:
+    """The built library of ``name``, after checking that it was compiled from
+    the sources the current generator writes (the reference's sample programs,
+    where the checkout is absent: from the generated source in the folder).
+    Raises LibraryStale otherwise; compiles nothing."""
+    if name in SAMPLES and not reference_present():
+        d = os.path.join(BUILD_DIR, name)
+    else:
+        d = regenerate(name)
+    out = lib_path(name)
+    if not os.path.exists(out):
+        raise LibraryStale("%s is not built; run __graft_entry__.build()" % out)
+    stamp_file = out + ".stamp"
+    if not os.path.exists(stamp_file) or open(stamp_file).read() != source_stamp(name, d):
+        raise LibraryStale("%s was built from other sources than the current generator writes; "
+                           "run __graft_entry__.build()" % out)
+    return out
+
+
+class LibraryStale(RuntimeError):
+    pass
+
+
+def builder_of(name: str):
+    """The build function of library ``name`` (program, batch, adjoint, side, sample)."""
+    if name in BATCHES:
+        return build_batch
+    if name in ADJOINTS:
+        return build_adjoint
+    if name.startswith("gs_side_"):
+        return build_side
+    if name in SAMPLES:
+        return build_sample
+    return build_one
+
+
 def build_all(names: Optional[List[str]] = None, force: bool = False, jobs: int = 0) -> List[str]:
     names = names or (list(PROGRAMS) + list(BATCHES) + list(ADJOINTS) + [side_name(n) for n in ENSEMBLE_SIDES]
                       + (list(SAMPLES) if reference_present() else []))
     jobs = jobs or min(8, os.cpu_count() or 4)
     outs = []
 
-    def builder(n):
-        if n in BATCHES:
-            return build_batch
-        if n in ADJOINTS:
-            return build_adjoint
-        if n.startswith("gs_side_"):
-            return build_side
-        if n in SAMPLES:
-            return build_sample
-        return build_one
-
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = {ex.submit(builder(n), n, force): n for n in names}
+        futs = {ex.submit(builder_of(n), n, force): n for n in names}
         for f in cf.as_completed(futs):
             outs.append(f.result())
     return outs

@@ -28,13 +28,20 @@ def load_lib(path, layout=None):
     return _LIBS[path]
 
 
-def hip_lib(name):
-    """The HIP library of a bundled program; built only when nothing is built yet."""
+def current_lib(name):
+    """Path of the built library ``name`` (any kind).  Where a GPU is visible
+    the library must have been built from the sources the current generator
+    writes (a stale one fails the test: build before running on the GPU);
+    without a GPU it is built when missing or stale."""
     from formura_amd import build
-    p = build.lib_path(name)
-    if not os.path.exists(p):
-        build.build_one(name)
-    return load_lib(p)
+    if has_gpu():
+        return build.check_current(name)
+    return build.builder_of(name)(name)
+
+
+def hip_lib(name):
+    """The HIP library of a bundled program (checked as ``current_lib``)."""
+    return load_lib(current_lib(name))
 
 
 def oracle_lib(name, layout, kind="strict"):

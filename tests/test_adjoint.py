@@ -24,9 +24,8 @@ THETA = [0.04, 0.06, 2e-5, 1e-5]
 
 
 def _stepper(name):
-    import os
-    if not os.path.exists(build.lib_path(name)):
-        build.build_adjoint(name)
+    from conftest import current_lib
+    current_lib(name)
     return A.Stepper(name)
 
 

@@ -23,9 +23,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _ens(name, sizes, params):
-    import os
-    if not os.path.exists(build.lib_path(name)):
-        build.build_batch(name)
+    from conftest import current_lib
+    current_lib(name)
     return B.Ensemble(name, sizes, params)
 
 

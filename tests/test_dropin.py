@@ -25,9 +25,8 @@ def _build_and_run(tmp_path, lib_dir, lib_name, header_dir):
 def test_user_main_links_unchanged(tmp_path):
     from formura_amd import build
     import build_oracle
-    hip_so = build.lib_path("heat3d_64")
-    if not os.path.exists(hip_so):
-        build.build_one("heat3d_64")
+    from conftest import current_lib
+    hip_so = current_lib("heat3d_64")
     hdr_dir = os.path.dirname(build.header_path("heat3d_64"))
     out_hip = _build_and_run(tmp_path, hdr_dir, "heat3d_64", hdr_dir)
     # the oracle as a library with the same soname pattern

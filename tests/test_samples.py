@@ -21,7 +21,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import load_lib
+from conftest import current_lib, load_lib
 from formura_amd import build as B, inputs
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -71,6 +71,7 @@ def test_sample_library_matches_fixture_bit_exact(name):
     path = B.lib_path(name)
     if not os.path.exists(path):
         pytest.skip("%s is built only where the formura checkout is present" % name)
+    path = current_lib(name)
     meta, want = fixture(name)
     print("sample %s: %d steps, grid %s" % (name, meta["steps"], meta["grid"]), flush=True)
     lib = load_lib(path)
@@ -131,6 +132,7 @@ def test_mixed_sample_against_fp64(name):
     path = B.lib_path(name + "_mixed")
     if not os.path.exists(path):
         pytest.skip("%s_mixed is built only where the formura checkout is present" % name)
+    path = current_lib(name + "_mixed")
     meta, want = fixture(name)
     print("mixed sample %s: %d steps" % (name, meta["steps"]), flush=True)
     lib = load_lib(path)
@@ -208,6 +210,7 @@ def test_reference_configuration_against_exact_solution(name):
     path = B.lib_path(name)
     if not os.path.exists(path):
         pytest.skip("%s is built only where the formura checkout is present" % name)
+    path = current_lib(name)
     with open(os.path.join(GOLDEN, "sample_%s.json" % name)) as f:
         meta = json.load(f)
     print("reference configuration %s: %d steps" % (name, meta["steps"]), flush=True)
