@@ -174,15 +174,45 @@ inline void timer_stop(Runtime &rt) {
   rt.last_ms = ms;
 }
 
+// Sum of n values of f into *dev_out through TOTALS_BLOCKS partial sums in
+// `part` (every slot written before it is read), enqueued on `s`.
 template <typename T>
-inline void field_total(Runtime &rt, const T *f, long long n, double *dev_out) {
-  if (rt.scratch_n < (size_t)TOTALS_BLOCKS) fatal("field_total: scratch not allocated");
-  hipLaunchKernelGGL(k_totals_partial<T>, dim3(TOTALS_BLOCKS), dim3(256), 0, rt.stream, f, n, rt.scratch);
-  hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(64), 0, rt.stream, rt.scratch, TOTALS_BLOCKS, dev_out);
+inline void field_total_with(double *part, const T *f, long long n, double *dev_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_totals_partial<T>, dim3(TOTALS_BLOCKS), dim3(256), 0, s, f, n, part);
+  hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(64), 0, s, part, TOTALS_BLOCKS, dev_out);
   FMR_CHECK(hipGetLastError());
 }
 
-inline double *scratch(Runtime &rt, size_t n) {
+template <typename T>
+inline void field_total(Runtime &rt, const T *f, long long n, double *dev_out) {
+  if (rt.scratch_n < (size_t)TOTALS_BLOCKS) fatal("field_total: scratch not allocated");
+  field_total_with(rt.scratch, f, n, dev_out, rt.stream);
+}
+
+// Partial sums of the allocation-free totals (Formura_HIP_Totals_Static): a
+// file-scope device array, one copy per GPU at its own address, held until the
+// process ends (8 KiB).  Every call on a GPU shares that GPU's copy, so calls
+// that overlap on one GPU (two streams, two host threads) race.
+static __device__ __align__(256) double scratch[TOTALS_BLOCKS];
+constexpr int MAX_DEVICES = 64;
+
+// Device address of `scratch` on the current device, looked up once per device
+// with that device current (the array's name in host code is a host-side
+// stand-in, not a device address).
+inline double *static_scratch() {
+  static double *ptr[MAX_DEVICES] = {};
+  int dev = 0;
+  FMR_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) fatal("static_scratch: device index out of range");
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    FMR_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptr[dev] = (double *)p;
+  }
+  return ptr[dev];
+}
+
+inline double *lazy_scratch(Runtime &rt, size_t n) {
   if (rt.scratch_n < n) {
     FMR_CHECK(hipStreamSynchronize(rt.stream));
     device_free(rt.scratch);

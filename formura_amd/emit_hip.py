@@ -375,6 +375,7 @@ extern "C" void Formura_Init(int *argc, char ***argv, Formura_Navi *n) {
 %(navi_init)s
   Formura_Setup(*n);
   fmr_rt::init(g_rt, STATE_BYTES);
+  (void)fmr_rt::static_scratch();  // device address of the totals' static scratch, once per device
 %(tmp_alloc)s
   fmr_rt::upload(g_rt, &%(inst)s);
 %(first)s
@@ -416,10 +417,34 @@ extern "C" long long Formura_HIP_Last_Launches(void) { return g_rt.launches; }
 
 extern "C" void Formura_HIP_Totals(double *out) {
   if (!g_rt.ready) fmr_rt::fatal("Formura_HIP_Totals called before Formura_Init");
-  double *res = fmr_rt::scratch(g_rt, fmr_rt::TOTALS_BLOCKS + %(nf)d) + fmr_rt::TOTALS_BLOCKS;
+  double *res = fmr_rt::lazy_scratch(g_rt, fmr_rt::TOTALS_BLOCKS + %(nf)d) + fmr_rt::TOTALS_BLOCKS;
 %(totals)s
   FMR_CHECK(hipMemcpyAsync(out, res, %(nf)d * sizeof(double), hipMemcpyDeviceToHost, g_rt.stream));
   FMR_CHECK(hipStreamSynchronize(g_rt.stream));
+}
+
+// Allocation-free totals: nfields fields of `cells` values each (element size
+// elem_bytes: 8 FP64, 4 FP32), back to back at device address `state` (a state
+// image such as Formura_HIP_State_Device()), summed as Formura_HIP_Totals does
+// into out[f] (device memory), enqueued on `stream` (0: the null stream).
+// The partial sums live in the file-scope device array of the runtime; no
+// memory is allocated or freed.
+extern "C" void Formura_HIP_Totals_Static(const void *state, int nfields, long long cells, int elem_bytes,
+                                          double *out, void *stream) {
+  if (nfields < 1 || nfields > FMR_TOTALS_STATIC_MAX_FIELDS)
+    fmr_rt::fatal("Formura_HIP_Totals_Static: nfields must be 1..64");
+  if (cells < 0 || cells > FMR_TOTALS_STATIC_MAX_CELLS)
+    fmr_rt::fatal("Formura_HIP_Totals_Static: cells must be 0..2^40");
+  if (elem_bytes != 4 && elem_bytes != 8) fmr_rt::fatal("Formura_HIP_Totals_Static: elem_bytes must be 4 or 8");
+  if (!state || !out) fmr_rt::fatal("Formura_HIP_Totals_Static: null pointer");
+  double *part = fmr_rt::static_scratch();
+  hipStream_t s = (hipStream_t)stream;
+  for (int f = 0; f < nfields; ++f) {
+    if (elem_bytes == 8)
+      fmr_rt::field_total_with(part, (const double *)state + (long long)f * cells, cells, out + f, s);
+    else
+      fmr_rt::field_total_with(part, (const float *)state + (long long)f * cells, cells, out + f, s);
+  }
 }
 
 extern "C" long long Formura_HIP_Positivity_Violations(void) { return fmr_rt::violations(g_rt); }

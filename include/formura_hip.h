@@ -47,6 +47,21 @@ long long Formura_HIP_Last_Launches(void);
 /* FP64 totals of every state field over the grid (out[f], fields in the order
  * of formura_data), summed on the device in a fixed order (deterministic). */
 void Formura_HIP_Totals(double *out);
+/* Allocation-free variant of Formura_HIP_Totals.  No counterpart in the
+ * reference (it has no device); for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Sums nfields (1..64) fields of `cells` (0..2^40) values
+ * each, of elem_bytes 8 (FP64) or 4 (FP32), stored back to back at the device
+ * address `state` (for instance Formura_HIP_State_Device()), into out[0..nfields)
+ * in device memory, in Formura_HIP_Totals's order (the same bits), enqueued on
+ * `stream` (a hipStream_t; 0 for the null stream); no host synchronisation.
+ * Partial sums use a file-scope device array of 8 KiB per GPU, held until the
+ * process ends; calls that overlap on one GPU race.  Needs no Formura_Init.
+ * Sizes outside the limits: message on stderr, exit(1). */
+#define FMR_TOTALS_STATIC_MAX_FIELDS 64
+#define FMR_TOTALS_STATIC_MAX_CELLS (1LL << 40)
+void Formura_HIP_Totals_Static(const void *state, int nfields, long long cells, int elem_bytes, double *out,
+                               void *stream);
 /* Mixed-precision libraries: FP64 square-root arguments (wave speeds from
  * pressure and density) found not > 0 since the library was loaded; 0 for
  * libraries without such checks. */
