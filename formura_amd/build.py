@@ -43,6 +43,9 @@ PROGRAMS: Dict[str, tuple] = {
     "hooks2d_32": ("hooks2d.fmr", "hooks2d_32.yaml", "auto"),
     "float2d_32": ("float2d.fmr", "float2d_32.yaml", "flat"),
     "float2d_32_tile": ("float2d.fmr", "float2d_32.yaml", {"kind": "tile2d", "tx": 16, "ty": 16, "k": 3}),
+    # FP32 state divided by a literal (double division, rounded to float) and by a float constant
+    "floatdiv2d_32": ("floatdiv2d.fmr", "float2d_32.yaml", "flat"),
+    "floatdiv2d_32_tile": ("floatdiv2d.fmr", "float2d_32.yaml", {"kind": "tile2d", "tx": 16, "ty": 16, "k": 3}),
     "hooks2d_32_tb": ("hooks2d.fmr", "hooks2d_32_tb.yaml", "auto"),
     "gs_1024": ("grayscott2d.fmr", "gs_1024.yaml", "auto"),
     "gs_2048": ("grayscott2d.fmr", "gs_2048.yaml", "auto"),
@@ -85,6 +88,19 @@ PROGRAMS: Dict[str, tuple] = {
     "gs_64_tile_t128": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 3, "threads": 128}),
     "gs_64_tile_92k": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 64, "ty": 64, "k": 6, "inplace": 1, "threads": 1024}),
     "gs_2048_wave": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "wave2d", "rows": 32, "k": 3}),
+    # the opt-in contract form (FMA contraction, reciprocal division; not bit-exact)
+    "gs_2048_contract": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "tile2d", "tx": 32, "ty": 32, "k": 4,
+                                                              "threads": 512, "contract": 1}),
+    "euler3d_256_contract": ("euler3d.fmr", "euler3d_256.yaml", {"kind": "staged3d", "ty": 4, "tz": 64, "cx": 32,
+                                                                  "threads": 512, "inline": "inner", "contract": 1}),
+    "gs_64_contract": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "contract": 1}),
+    # two adjacent points per work-item
+    "gs_64_tile_pair": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4,
+                                                           "threads": 128, "pair": 1}),
+    "gs_72x120_tile_pair": ("grayscott2d.fmr", "gs_72x120.yaml", {"kind": "tile2d", "tx": 24, "ty": 40, "k": 3,
+                                                                   "pair": 1}),
+    "gs_2048_tile_pair": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "tile2d", "tx": 32, "ty": 32, "k": 4,
+                                                               "threads": 512, "pair": 1}),
     # mixed precision: FP32 flux arithmetic, FP64 wave speeds, positivity checks and update
     "euler3d_16_mixed": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),
     "euler3d_32_mixed": ("euler3d.fmr", "euler3d_32.yaml", {"kind": "flat", "inline": 1, "mixed": 1}),

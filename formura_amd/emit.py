@@ -248,7 +248,27 @@ class Body:
                 self.lines.append("%s%s %s = (%s%d - %s%d) + (%s%d - %s%d);"
                                   % (const_kw, ct, name, prefix, pa, prefix, pc, prefix, pb, prefix, pc))
                 continue
-            if ((real or recip or ct == "float") and k == "Binop" and m.inst[1] == "/" and m.inst[3] in vals
+            if (ct == "float" and not (real or recip or prec) and k == "Binop" and m.inst[1] == "/"
+                    and m.inst[3] in vals and m.inst[2] not in vals and divc and _divc_ok(vals[m.inst[3]])):
+                # a float divided by a constant: the reference's C divides in
+                # double (the literal is a double, Templates.hs:475) and rounds
+                # to float on assignment; that double quotient (fmr_divc, the
+                # correctly rounded one) rounded to float is the correctly
+                # rounded float quotient (53 >= 2*24 + 2: no double rounding)
+                b = float(vals[m.inst[3]])
+                if ctypes.get(m.inst[3]) == "float":
+                    b = float(_f32(b))  # a float constant holds its value rounded to float
+                src = "(double)%s%d" % (prefix, m.inst[2])
+                mant, _ = math.frexp(abs(b))
+                if mant == 0.5:
+                    rhs = "%s*%s" % (src, show_double(1.0 / b))
+                else:
+                    fn = "fmr_divc1" if _divc_one_step(abs(b)) else "fmr_divc"
+                    sg = "" if b > 0 else "-"
+                    rhs = "%s%s(%s, %s, %s)" % (sg, fn, src, show_double(abs(b)), show_double(1.0 / abs(b)))
+                self.lines.append("%s%s %s = (float)(%s);" % (const_kw, ct, name, rhs))
+                continue
+            if ((real or recip or (ct == "float" and prec)) and k == "Binop" and m.inst[1] == "/" and m.inst[3] in vals
                     and m.inst[2] not in vals):
                 rc = 1.0 / vals[m.inst[3]] if vals[m.inst[3]] else float("inf")
                 lit = _const_literal(_f32(rc), "float") if ct == "float" else show_double(rc)

@@ -530,6 +530,7 @@ extern "C" void Formura_Batch_Get_State(FormuraBatch* b, int i, double* fields) 
 
 // drifting-frame offsets of instance i after the steps run so far
 extern "C" void Formura_Batch_Offsets(FormuraBatch* b, int i, int* off) {
+  if (i < 0 || i >= b->ninst) fmr_rt::fatal("Formura_Batch_Offsets: bad instance");
   const Inst& I = b->inst[i];
   const long long sh = (long long)SLEEVE * b->steps_done[i];
   off[0] = (int)(((-sh) %% I.n1 + I.n1) %% I.n1);

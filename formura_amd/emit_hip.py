@@ -66,6 +66,11 @@ class LibEmitter:
         # arguments and the update chain (emit.mixed_precision), decided per
         # node before any inlining
         self.mixed = bool(isinstance(spec, dict) and spec.get("mixed"))
+        # contract: an opt-in FP64 form that is not bit-exact -- FMA contraction
+        # across the instruction list and division by a constant as the
+        # product with its reciprocal (checked against the oracle to a stated
+        # tolerance, not bit for bit)
+        self.contract = bool(isinstance(spec, dict) and spec.get("contract"))
         self._prec = {k: mixed_precision(n) for k, n in mm.step.items()} if self.mixed else None
         storage = None
         if self.mixed and not self.inlined:
@@ -170,7 +175,7 @@ class LibEmitter:
             load, tmp_load = loaders([0] * dim)
             precs = [self.prec_of(plan.mm[k]) or {} for k in grp] if self.mixed else None
             node, roots, mprec = _merge_nodes([plan.mm[k] for k in grp], precs)
-            body = Body(node, load, tmp_load, lambda ax: "0", lambda n, v: "", "a", prec=mprec)
+            body = Body(node, load, tmp_load, lambda ax: "0", lambda n, v: "", "a", prec=mprec, recip=self.contract)
             body.lines += ["t_%d[t] = (%s)a%d;" % (k, plan.temp_ctype(k), roots[i]) for i, k in enumerate(grp)]
             deps = temp_group_inputs(plan, grp)
             outs_ = "".join(", %s* __restrict__ t_%d" % (plan.temp_ctype(k), k) for k in grp)
@@ -194,7 +199,7 @@ class LibEmitter:
             def index(ax):
                 return "(%s+off_%d)" % (ivs[ax], ax + 1)
 
-            body = Body(node, load, tmp_load, index, store, "s%d_" % k, prec=self.prec_of(node))
+            body = Body(node, load, tmp_load, index, store, "s%d_" % k, prec=self.prec_of(node), recip=self.contract)
             lines.append("{")
             lines += ["  " + l for l in body.lines]
             lines.append("}")
@@ -273,6 +278,7 @@ class LibEmitter:
         desc = lib_description(om, self.name, self.schedule.name)
         L = ["// Generated by formura_amd from %s. Do not edit." % self.name,
              "#include \"formura_hip_rt.h\"",
+             "#pragma clang fp contract(fast)" if self.contract else "// no FMA contraction: every operation rounded as written",
              "extern \"C\" {",
              "#include \"%s.h\"" % self.name,
              "}",
@@ -406,7 +412,13 @@ extern "C" void Formura_Finalize() {
   fmr_rt::shutdown(g_rt);
 }
 
-extern "C" void Formura_HIP_Set_Resident(int on) { g_rt.resident = on ? 1 : 0; }
+// Leaving residency downloads the device state into formura_data, so that the
+// next Formura_Forward (which uploads formura_data) continues from it.
+extern "C" void Formura_HIP_Set_Resident(int on) {
+  const int now = on ? 1 : 0;
+  if (g_rt.ready && g_rt.resident && !now) fmr_rt::download(g_rt, &%(inst)s);
+  g_rt.resident = now;
+}
 extern "C" void Formura_HIP_Upload(void) { if (g_rt.ready) fmr_rt::upload(g_rt, &%(inst)s); }
 extern "C" void Formura_HIP_Download(void) { if (g_rt.ready) fmr_rt::download(g_rt, &%(inst)s); }
 extern "C" const char *Formura_HIP_Describe(void) { return kDescription; }
@@ -540,10 +552,15 @@ class Tile2DSchedule:
 
     name = "tile2d"
 
-    def __init__(self, tx, ty, k, inplace=False, threads=256):
+    def __init__(self, tx, ty, k, inplace=False, threads=256, pair=False):
         self.tx, self.ty, self.k = tx, ty, k
         # work-items per workgroup (a level's points are dealt round-robin)
         self.threads = int(threads)
+        # pair: each work-item evaluates two adjacent points of a row; a level's
+        # work-items form a fixed 2-D grid (P pairs per row, NTH/P rows per
+        # pass), so a point's LDS offset advances by a constant per pass
+        # instead of being decomposed from a flat index
+        self.pair = bool(pair)
         # in place: one LDS buffer per field; a level's values go to registers
         # and overwrite the previous level after a barrier (half the LDS, so
         # more workgroups per CU, for one more barrier per level)
@@ -578,7 +595,7 @@ class Tile2DSchedule:
                 return "(%s + ib%d)" % ("gx" if ax == 0 else "gy", ax)
 
             b = Body(node, load, lambda *a: None, index if tbi else (lambda ax: None), store, "s%d_" % k,
-                     prec=em.prec_of(node))
+                     prec=em.prec_of(node), recip=em.contract)
             body_lines.append("{")
             body_lines += ["  " + l for l in b.lines]
             body_lines.append("}")
@@ -603,6 +620,17 @@ class Tile2DSchedule:
         refs_o = ", ".join("o%d" % i for i in range(len(fields)))
         st_lds = " ".join("B%d[dst][o] = o%d;" % (i, i) for i in range(len(fields)))
         st_glb = " ".join("FLD%d(st_out)[g] = o%d;" % (i, i) for i in range(len(fields)))
+        if self.pair and not self.inplace and not tbi:
+            decl_p = " ".join("%s p%d;" % (ct, i) for i, (n, ct) in enumerate(fields))
+            refs_p = ", ".join("p%d" % i for i in range(len(fields)))
+            srcs0 = ", ".join("B%d[src]" % i for i in range(len(fields)))
+            st_lds2 = " ".join("B%d[dst][o] = o%d; B%d[dst][o + 1] = p%d;" % (i, i, i, i) for i in range(len(fields)))
+            st_glb2 = " ".join("FLD%d(st_out)[g] = o%d; FLD%d(st_out)[g + 1] = p%d;" % (i, i, i, i)
+                               for i in range(len(fields)))
+            out.append(TILE2D_PAIR % dict(TX=TX, TY=TY, NTH=self.threads, lds=lds, load_tile=load_tile,
+                                          decl_o=decl_o + " " + decl_p, srcs=srcs0, refs_o=refs_o, refs_p=refs_p,
+                                          st_lds=st_lds2, st_glb=st_glb2))
+            return out
         if self.inplace:
             out.append(TILE2D_INPLACE % dict(
                 TX=TX, TY=TY, NTH=self.threads, lds=lds, load_tile=load_tile, decl_o=decl_o, refs_o=refs_o,
@@ -682,6 +710,51 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ 
                 "    const int k = steps >= %d ? %d : (int)steps;\n"
                 "    run_step_tile(k, g_rt.cur, g_rt.nxt, g_rt.stream); g_rt.swap();\n"
                 "    g_rt.launches += 1; steps -= k;\n  }\n}" % (self.k, self.k))
+
+
+TILE2D_PAIR = """
+template <int K>
+__global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
+  constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
+  constexpr int NTX = N1 / TX, NTY = N2 / TY, NT = NTX * NTY;
+%(lds)s
+  // XCD-aware tile order: workgroup ids are dealt round-robin to the 8 XCDs, so
+  // give each XCD a contiguous band of tiles (shared halos stay in its L2).
+  const int b = blockIdx.x;
+  const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
+  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+%(load_tile)s
+  __syncthreads();
+#pragma unroll
+  for (int j = 1; j <= K; ++j) {
+    const int src = (j - 1) & 1, dst = j & 1;
+    // level j covers rows and columns [lo, RX) x [lo, RY): W = RY - lo columns
+    // (even: TY and lo are), P = W/2 pairs per row, R rows per pass
+    constexpr int lo0 = 2 * SLEEVE;
+    static_assert((RY - lo0) %% 2 == 0 && (RY - lo0) / 2 <= %(NTH)d, "a row's pairs must fit the workgroup");
+    const int lo = lo0 * j, W = RY - lo, HH = RX - lo, P = W / 2, R = %(NTH)d / P;
+    const int t = threadIdx.x;
+    if (t < R * P) {
+      const int pr = t / P, pc = t - pr * P;
+      const int c = lo + 2 * pc;
+      int o = (lo + pr) * RY + c;
+      for (int r = lo + pr; r < RX; r += R, o += R * RY) {
+        %(decl_o)s
+        pt_step<RY>(%(srcs)s, o, %(refs_o)s);
+        pt_step<RY>(%(srcs)s, o + 1, %(refs_p)s);
+        if (j < K) {
+          %(st_lds)s
+        } else {
+          const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+          %(st_glb)s
+        }
+      }
+    }
+    (void)HH;
+    if (j < K) __syncthreads();
+  }
+}
+"""
 
 
 TILE2D_INPLACE = """
@@ -819,7 +892,7 @@ class Wave2DSchedule:
                     return "n%d = %s;" % (em.field_index[fname], v)
 
                 b = Body(node, load, lambda *a: None, lambda ax: None, store, "%s%d_" % (prefix, k),
-                         prec=em.prec_of(node))
+                         prec=em.prec_of(node), recip=em.contract)
                 lines.append("{")
                 lines += ["  " + l for l in b.lines]
                 lines.append("}")
@@ -1082,7 +1155,8 @@ class Staged3DSchedule:
                 fi = em.field_index[field]
                 return "S%d[sl_%d_%s][so + (%d)*%d + (%d)]" % (fi, gi, str(c[0]).replace("-", "m"), c[1], st["SRZ"], c[2])
 
-            b = Body(node, load, lambda *a: None, lambda ax: None, lambda n, v: "", "g%d_" % gi, prec=mprec)
+            b = Body(node, load, lambda *a: None, lambda ax: None, lambda n, v: "", "g%d_" % gi, prec=mprec,
+                     recip=em.contract)
             kw = "if" if gi == 0 else "} else if"
             # planes of this group computed before iteration x0 - (DG - 1) are never read
             lines = ["%s (gidx < %d) {  // group %d: temporaries %s, read at x offsets %d..%d"
@@ -1124,7 +1198,7 @@ class Staged3DSchedule:
             def store(fname, v):
                 return "FLD%d(st_out)[gout] = %s;" % (em.field_index[fname], v)
 
-            b = Body(node, sload, tload, lambda ax: None, store, "u%d_" % k, prec=em.prec_of(node))
+            b = Body(node, sload, tload, lambda ax: None, store, "u%d_" % k, prec=em.prec_of(node), recip=em.contract)
             sbody += ["{"] + ["  " + l for l in b.lines] + ["}"]
         slot_lines = []
         for kind, o in sorted(xslots):
@@ -1253,7 +1327,7 @@ class Stream3DSchedule:
                 return "(g%d + ib%d)" % (ax, ax)
 
             b = Body(node, load, lambda *a: None, index if tbi else (lambda ax: None), store, "s%d_" % k,
-                     prec=em.prec_of(node))
+                     prec=em.prec_of(node), recip=em.contract)
             body_lines.append("{")
             body_lines += ["  " + l for l in b.lines]
             body_lines.append("}")
@@ -1442,7 +1516,13 @@ def schedule_from_spec(em, spec: dict):
         args = (int(spec["tx"]), int(spec["ty"]), int(spec["k"]))
         if not _tile2d_ok(em, *args, inplace=bool(spec.get("inplace", 0))):
             raise EmitError("tile2d %r does not apply to %s" % (spec, em.name))
-        return Tile2DSchedule(*args, inplace=bool(spec.get("inplace", 0)), threads=int(spec.get("threads", 256)))
+        if spec.get("pair") and (int(spec["ty"]) % 2 or spec.get("inplace") or getattr(em, "tb_index", False)
+                                 or (int(spec["ty"]) + 2 * (int(spec["k"]) - 1) * em.step.sleeve) // 2
+                                 > int(spec.get("threads", 256))):
+            raise EmitError("tile2d pairs need an even tile width, a row's pairs within the workgroup, "
+                            "ping-pong buffers and no index reads")
+        return Tile2DSchedule(*args, inplace=bool(spec.get("inplace", 0)), threads=int(spec.get("threads", 256)),
+                              pair=bool(spec.get("pair", 0)))
     if kind == "wave2d":
         args = (int(spec["rows"]), int(spec["k"]))
         if not _wave2d_ok(em, *args):
@@ -1511,6 +1591,8 @@ def candidate_specs(em) -> List[dict]:
                             if nth != 256:
                                 spec["threads"] = nth
                             out.append(spec)
+                            if not inplace and ty % 2 == 0 and not getattr(em, "tb_index", False):
+                                out.append(dict(spec, pair=1))
         for rows in (8, 16, 32, 64):
             for k in (1, 2, 3, 4):
                 if _wave2d_ok(em, rows, k):

@@ -25,7 +25,9 @@ extern "C" {
 #endif
 
 /* 1: the state stays in HBM between calls; Formura_Forward no longer copies
- * formura_data.  0 (default): every Formura_Forward syncs both ways. */
+ * formura_data.  0 (default): every Formura_Forward syncs both ways.  Going
+ * from 1 to 0 downloads the device state into formura_data first (host
+ * writes made while resident and not uploaded are overwritten). */
 void Formura_HIP_Set_Resident(int on);
 /* Copy host formura_data to the device state / the device state to host. */
 void Formura_HIP_Upload(void);

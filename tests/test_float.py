@@ -72,6 +72,57 @@ def test_float_library_bit_exact(name, steps):
     np.testing.assert_array_equal(h.field("q"), q["q"])
 
 
+def div_program():
+    return generate(os.path.join(B.PROG_DIR, "floatdiv2d.fmr"), os.path.join(B.PROG_DIR, "float2d_32.yaml"),
+                    "floatdiv2d", "flat")[2]
+
+
+def hand_div_step(q):
+    """s/3 + (q - q[i,j-1])/10 - q/a with s = (q[i+1,j] + q[i-1,j]) + q[i,j+1]:
+    float sums and differences; each quotient by a literal (3, 10, and a =
+    7/10, a Rational constant) taken in double and rounded to float."""
+    f32, f64 = np.float32, np.float64
+    c = np.roll(q, (1, 1), axis=(0, 1))          # q[i-1, j-1]: the centre
+    xp = np.roll(q, (0, 1), axis=(0, 1))
+    xm = np.roll(q, (2, 1), axis=(0, 1))
+    yp = np.roll(q, (1, 0), axis=(0, 1))
+    ym = np.roll(q, (1, 2), axis=(0, 1))
+    s = (xp + xm) + yp
+    t1 = (s.astype(f64) / 3.0).astype(f32)
+    t2 = ((c - ym).astype(f64) / 10.0).astype(f32)
+    t3 = (c.astype(f64) / (7.0 / 10.0)).astype(f32)
+    return ((t1 + t2) - t3).astype(f32)
+
+
+def test_typed_interpreter_divides_like_c():
+    mm = div_program()
+    q = np.random.default_rng(6).random((32, 48)).astype(np.float32)
+    a, b = {"q": q}, q
+    for _ in range(5):
+        a = mm_interp.step_typed(mm.step, 1, a)
+        b = hand_div_step(b)
+        np.testing.assert_array_equal(a["q"], b)
+    # the product with the float reciprocal is a different number in many cells
+    bad = (q.astype(np.float32) * np.float32(1.0 / 3.0)).astype(np.float32)
+    good = (q.astype(np.float64) / 3.0).astype(np.float32)
+    assert np.count_nonzero(bad != good) > 100
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,steps", [("floatdiv2d_32", 6), ("floatdiv2d_32_tile", 9)])
+def test_float_division_bit_exact(name, steps):
+    print("%s: %d steps" % (name, steps), flush=True)
+    mm = div_program()
+    h = hip_lib(name)
+    h.init()
+    q = {"q": np.random.default_rng(10).random((32, 48)).astype(np.float32)}
+    h.set_state(q)
+    h.forward(steps)
+    for _ in range(steps):
+        q = mm_interp.step_typed(mm.step, 1, q)
+    np.testing.assert_array_equal(h.field("q"), q["q"])
+
+
 def kahan_program():
     return generate(os.path.join(B.PROG_DIR, "heat3d_kahan.fmr"), os.path.join(B.PROG_DIR, "heat3d_kahan_16.yaml"),
                     "heat3d_kahan", "flat")[2]

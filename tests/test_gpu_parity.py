@@ -103,6 +103,8 @@ def test_resident_multi_step_launches(name, steps):
     ("heat3d_32_stream512", "heat3d_32", 9, True),    # stream3d with 512 work-items per workgroup
     ("gs_64_tile_92k", "gs_64", 25, True),            # 92 KiB of LDS in one workgroup (gfx950 allows 160)
     ("gs_2048_wave", "gs_2048", 31, True),            # the same at full size (strips overlap at the last column)
+    ("gs_64_tile_pair", "gs_64", 39, True),           # two adjacent points per work-item, 4 steps per launch
+    ("gs_2048_tile_pair", "gs_2048", 41, True),       # the same at full size, 32 x 32 tiles
 ])
 def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     seed = (lambda lib: inputs.heat_field(13, lib.grid)) if name.startswith("heat") else _gs_seed
@@ -115,6 +117,7 @@ def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     ("gs_72x120", "gs_72x120", 37, True),
     ("gs_72x120_tile", "gs_72x120", 41, True),              # 24 x 40 tiles, 3 steps per launch + remainder
     ("gs_72x120_wave", "gs_72x120", 29, True),              # register windows on a 72-row grid
+    ("gs_72x120_tile_pair", "gs_72x120", 35, True),         # 24 x 40 tiles, two points per work-item
     ("heat3d_24x40x56", "heat3d_24x40x56", 13, False),      # default (flat) schedule
     ("heat3d_24x40x56_stream", "heat3d_24x40x56", 15, True),  # 8 x 8 columns along a 24-plane axis
     ("heat3d_24x40x56_tb", "heat3d_24x40x56_tb", 5, False),  # 2 steps per Forward, blocks 7 x 11 x 6
@@ -174,4 +177,24 @@ def test_euler_hllc_bit_exact(name, oracle, steps, resident):
 
 def test_euler_hllc_full_size():
     o, h = _run_pair("euler3d_256", "euler3d_256", _euler_seed, 2, resident=True)
+    _assert_same(o, h)
+
+
+def test_leaving_residency_keeps_device_progress():
+    """Resident steps, then Set_Resident(0) without an explicit download, then
+    plain Forward calls (each uploads formura_data): the run continues from the
+    device state, as if all steps had been plain calls."""
+    o = oracle_lib("gs_64", layout_of("gs_64"))
+    h = hip_lib("gs_64")
+    for lib in (o, h):
+        lib.init()
+    st = inputs.grayscott_perturb(o.field("u").copy(), o.field("v").copy(), 8)
+    o.set_state(st)
+    h.set_state(st)
+    h.set_resident(True)
+    h.upload()
+    h.forward_steps(5)
+    h.set_resident(False)
+    h.forward(3)
+    o.forward(8)
     _assert_same(o, h)
