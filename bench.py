@@ -82,6 +82,12 @@ def run_gpu(name: str, steps: int, warmup: int, bytes_per_lup: float, seed: int 
     }
 
 
+# the FP32 forms of the batched ensemble: f32k (Kahan-compensated accumulation),
+# f32c (and the increment in float-float FP32 from the FP32 state), f32x (the
+# increment in float-float FP32 from the compensated state)
+FP32_FORMS = ("f32k", "f32c", "f32x")
+
+
 def run_ensemble(count: int, steps: int):
     """The batched FP32 Gray-Scott ensemble in its two forms (f32k: Kahan-compensated
     accumulation; f32c: and the increment in float-float FP32) and its FP64 twin on
@@ -98,7 +104,7 @@ def run_ensemble(count: int, steps: int):
     out = {"instances": count, "cells": cells, "steps": steps,
            "sides": [min(s[0] for s in sizes), max(s[0] for s in sizes)]}
     ens = {}
-    for name in ("gs_batch_f32k", "gs_batch_f32c", "gs_batch_f64"):
+    for name in ("gs_batch_f32k", "gs_batch_f32c", "gs_batch_f32x", "gs_batch_f64"):
         e = B.Ensemble(name, sizes, params)
         for i, st in enumerate(inits):
             e.set_state(i, st)
@@ -110,7 +116,7 @@ def run_ensemble(count: int, steps: int):
                      "tile": e.desc["tile"], "steps_per_launch": e.desc["steps_per_launch"]}
         ens[name] = e
     all_errs = {}
-    for prec in ("f32k", "f32c"):
+    for prec in FP32_FORMS:
         errs, rms = [], {"u": 0.0, "v": 0.0}
         for i in range(count):
             x, y = ens["gs_batch_f64"].get_state(i), ens["gs_batch_" + prec].get_state(i)
@@ -144,10 +150,14 @@ def run_ensemble(count: int, steps: int):
         sens.append((num / den) ** 0.5)
     sens = np.array(sens)
     out["f64_perturbed_2^-24_rel_l2"] = {
-        "median": float(np.median(sens)), "instances_le_1e-5": int((sens <= 1e-5).sum()),
-        "f32k_above_1e-5_where_f64_control_le_1e-5": int(((errs > 1e-5) & (sens <= 1e-5)).sum()),
-        "f32k_le_1e-5_where_f64_control_above_1e-5": int(((errs <= 1e-5) & (sens > 1e-5)).sum()),
-        "f32c_above_1e-5_where_f64_control_le_1e-5": int(((all_errs["f32c"] > 1e-5) & (sens <= 1e-5)).sum())}
+        "median": float(np.median(sens)), "instances_le_1e-5": int((sens <= 1e-5).sum())}
+    for prec in FP32_FORMS:
+        e = all_errs[prec]
+        out["f64_perturbed_2^-24_rel_l2"].update({
+            prec + "_above_1e-5_where_f64_control_le_1e-5": int(((e > 1e-5) & (sens <= 1e-5)).sum()),
+            prec + "_le_1e-5_where_f64_control_above_1e-5": int(((e <= 1e-5) & (sens > 1e-5)).sum())})
+    out["per_instance"] = {"side": [int(a) for a, _ in sizes], "control": [float(x) for x in sens],
+                           **{prec: [float(x) for x in all_errs[prec]] for prec in FP32_FORMS}}
     for e in ens.values():
         e.close()
     return out
@@ -168,7 +178,7 @@ def run_fp32_single(steps: int = 10000, seed: int = 7):
     params = np.array([[0.04, 0.06]])
     out = {"grid": [2048, 2048], "steps": steps}
     res = {}
-    for name in ("gs_batch_f64", "gs_batch_f32k", "gs_batch_f32c"):
+    for name in ("gs_batch_f64",) + tuple("gs_batch_" + p for p in FP32_FORMS):
         e = B.Ensemble(name, [(2048, 2048)], params)
         e.set_state(0, st)
         e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"])  # warm-up, captures the graph
@@ -186,7 +196,7 @@ def run_fp32_single(steps: int = 10000, seed: int = 7):
     res["control"] = e.get_state(0)
     e.close()
     x = res["gs_batch_f64"]
-    for prec in ("f32k", "f32c", "control"):
+    for prec in FP32_FORMS + ("control",):
         y = res["gs_batch_" + prec] if prec != "control" else res["control"]
         out[prec + "_error_vs_fp64" if prec != "control" else "f64_perturbed_2^-24_vs_fp64"] = {
             f: {"rel_l2": float(np.sqrt(np.sum((y[f] - x[f]) ** 2) / np.sum(x[f] ** 2))),
@@ -640,9 +650,9 @@ def main():
             sizes, _, _ = B.grayscott_ensemble(args.ensemble)
             pi = run_per_instance_fp64(sizes, args.ensemble_steps)
             ens["per_instance_fp64"] = pi
-            ens["fp32_batched_vs_per_instance_fp64"] = ens["gs_batch_f32k"]["glups"] / pi["glups"]
-            ens["fp32c_batched_vs_per_instance_fp64"] = ens["gs_batch_f32c"]["glups"] / pi["glups"]
-            ens["fp32_batched_vs_plain_fp64_2048"] = ens["gs_batch_f32k"]["glups"] / r["glups"]
+            for prec in FP32_FORMS:
+                ens[prec + "_batched_vs_per_instance_fp64"] = ens["gs_batch_" + prec]["glups"] / pi["glups"]
+                ens[prec + "_batched_vs_plain_fp64_2048"] = ens["gs_batch_" + prec]["glups"] / r["glups"]
             line["ensemble"] = ens
         except Exception as e:
             line["ensemble"] = {"error": str(e)[:300]}

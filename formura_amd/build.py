@@ -185,6 +185,7 @@ BATCHES: Dict[str, tuple] = {
     "gs_batch_f64": ("grayscott2d.fmr", "gs_64.yaml", "f64", ["f", "k"], 16, 32, 3),
     "gs_batch_f32k": ("grayscott2d.fmr", "gs_64.yaml", "f32k", ["f", "k"], 16, 32, 4),
     "gs_batch_f32c": ("grayscott2d.fmr", "gs_64.yaml", "f32c", ["f", "k"], 16, 32, 4),
+    "gs_batch_f32x": ("grayscott2d.fmr", "gs_64.yaml", "f32x", ["f", "k"], 32, 32, 4),
     # 2-D diffusion (dx = 1/64 from the template config, dt = dx^2/10), no lifted parameters
     "diff_batch_f64": ("diffusion2d.fmr", "diffusion2d_64.yaml", "f64", [], 32, 32, 4),
     "diff_batch_f32k": ("diffusion2d.fmr", "diffusion2d_64.yaml", "f32k", [], 32, 32, 4),

@@ -45,7 +45,8 @@ from . import syntax as S
 from .emit import Body, tile_load, EmitError, GraphPlan, state_fields
 from .manifest import calc_range
 
-PRECISIONS = {"f64": ("double", False), "f32k": ("float", True), "f32c": ("float", True)}
+PRECISIONS = {"f64": ("double", False), "f32k": ("float", True), "f32c": ("float", True),
+              "f32x": ("float", True)}
 
 
 def _f32(x: float) -> float:
@@ -74,7 +75,8 @@ def _flit(x: float) -> str:
     return "%.9ef" % x
 
 
-def ff_increment(node, body: Body, root: int, pidx: Dict[str, int], pre: str, stencil_plain: bool = True):
+def ff_increment(node, body: Body, root: int, pidx: Dict[str, int], pre: str, stencil_plain: bool = True,
+                 comp_load=None):
     """Float-float evaluation of the sub-DAG of ``node`` below micro ``root`` (the
     increment dq of an update q + dq).  ``body`` is the node's plain FP32 Body
     (its values ``<prefix><j>`` and folded constants ``body.vals``); ``pidx``
@@ -89,7 +91,12 @@ def ff_increment(node, body: Body, root: int, pidx: Dict[str, int], pre: str, st
       x / c: x * (hi, lo) of 1/c (c a constant, 1/c rounded in FP64)
     A pair is left unnormalized between operations (|lo| stays near ulp(hi));
     the result is normalized once (Fast2Sum).  Operations outside + - * / and
-    unary minus (square roots, comparisons, selections) take the hi parts."""
+    unary minus (square roots, comparisons, selections) take the hi parts.
+
+    ``comp_load(field, cursor)``: a C expression of the Kahan compensation c of
+    a state load; the load is then the pair (q, -c) -- the compensated value
+    the accumulation carries -- and every value is a pair (no plain stencil
+    part), so the increment is computed from the state to about 2^-48."""
     micro = node.micro
     # the plain FP32 value of micro j is named like the Body's: prefix + j
     prefix = body.result.rsplit("_", 1)[0] + "_"
@@ -138,7 +145,8 @@ def ff_increment(node, body: Body, root: int, pidx: Dict[str, int], pre: str, st
             curs[j] = curs[a] | curs[b]
         else:
             lin[j], curs[j] = False, set().union(*[curs.get(x, set()) for x in m.inst[2:] if isinstance(x, int)])
-    plain = {j for j in need if j not in const and lin[j] and len(curs[j]) >= 2} if stencil_plain else set()
+    plain = ({j for j in need if j not in const and lin[j] and len(curs[j]) >= 2}
+             if stencil_plain and comp_load is None else set())
     lines = []
     pair = {}
 
@@ -153,6 +161,8 @@ def ff_increment(node, body: Body, root: int, pidx: Dict[str, int], pre: str, st
     def get(x):
         if x in const:
             return cpair(x)
+        if micro[x].inst[0] == "LoadCursorStatic" and comp_load is not None:
+            return "%s%d" % (prefix, x), "(-%s)" % comp_load(micro[x].inst[2], micro[x].inst[1])
         if x in plain or micro[x].inst[0] == "LoadCursorStatic":
             return "%s%d" % (prefix, x), None
         return pair[x]
@@ -235,7 +245,9 @@ class BatchEmitter:
             raise EmitError("unknown precision %r" % precision)
         self.real, self.kahan = PRECISIONS[precision]
         self.precision = precision
-        self.compensated = precision == "f32c"
+        self.compensated = precision in ("f32c", "f32x")
+        # f32x: the increment reads the compensated state (q, -c) as pairs
+        self.comp_loads = precision == "f32x"
         self.params = list(params)
         self.tx, self.ty, self.k = tx, ty, k
         self.G = graph_launches + (graph_launches % 2)
@@ -284,7 +296,11 @@ class BatchEmitter:
             lines += ["  " + l for l in body.lines]
             if self.compensated:
                 q, dq = self._update_ids(node, fname, base)
-                fl, hi, lo = ff_increment(node, body, dq, {p: i for i, p in enumerate(self.params)}, "c%d_" % k)
+
+                def cload(field, c, base=base):
+                    return "clds%d[o + (%d)*RY + (%d)]" % (self.fi[field], c[0] + base, c[1] + base)
+                fl, hi, lo = ff_increment(node, body, dq, {p: i for i, p in enumerate(self.params)}, "c%d_" % k,
+                                          comp_load=cload if self.comp_loads else None)
                 lines += ["  " + l for l in fl]
                 lines.append("  old%d = s%d_%d; inc%d = %s; incl%d = %s;" % (fi, k, q, fi, hi, fi, lo))
             elif self.kahan:
@@ -294,6 +310,8 @@ class BatchEmitter:
                 lines.append("  old%d = 0; inc%d = s%d_%d;" % (fi, fi, k, node.micro[-1].inst[2]))
             lines.append("}")
         params = ", ".join("const %s* __restrict__ lds%d" % (real, i) for i in range(nf))
+        if self.comp_loads:
+            params += "".join(", const %s* __restrict__ clds%d" % (real, i) for i in range(nf))
         outs = ", ".join("%s& old%d, %s& inc%d" % (real, i, real, i) for i in range(nf))
         if self.compensated:
             outs += "".join(", float& incl%d" % i for i in range(nf))
@@ -310,12 +328,29 @@ class BatchEmitter:
                                           "+ wrapn(ty0 - H + c, I.n2);"], pairs, real, "ld", indent="    ",
                               threads=self.threads)
         srcs = ", ".join("B%d[sb]" % i for i in range(nf))
+        if self.comp_loads:
+            srcs += "".join(", C%d[sb]" % i for i in range(nf))
         decl = " ".join("%s old%d, inc%d;" % (real, i, i) for i in range(nf))
         refs = ", ".join("old%d, inc%d" % (i, i) for i in range(nf))
         if self.compensated:
             decl += " " + " ".join("float incl%d;" % i for i in range(nf))
             refs += "".join(", incl%d" % i for i in range(nf))
-        if self.kahan:
+        if self.comp_loads:
+            # f32x: the state is the float-float pair (q, -c) and the update a
+            # float-float sum of (q, -c) and the increment (dq_hi, dq_lo):
+            # TwoSum of the high parts, the low parts added, then Fast2Sum
+            # (kt - kc is the new value to about 2^-48)
+            upd = "\n".join(
+                ("        const float ks%(i)d = old%(i)d + inc%(i)d;\n"
+                 "        const float kb%(i)d = ks%(i)d - old%(i)d;\n"
+                 "        const float kl%(i)d = ((old%(i)d - (ks%(i)d - kb%(i)d)) + (inc%(i)d - kb%(i)d))"
+                 " + (incl%(i)d - C%(i)d[sb][o + CB]);\n"
+                 "        const float kt%(i)d = ks%(i)d + kl%(i)d;\n"
+                 "        const float kc%(i)d = (kt%(i)d - ks%(i)d) - kl%(i)d;") % {"i": i} for i in range(nf))
+            st_lds = " ".join("B%d[db][o] = kt%d; C%d[db][o] = kc%d;" % (i, i, i, i) for i in range(nf))
+            st_glb = " ".join("dst[%dLL * plane + g] = kt%d; dst[%dLL * plane + g] = kc%d;" % (i, i, nf + i, i)
+                              for i in range(nf))
+        elif self.kahan:
             # Kahan: ky = dq - c, kt = q + ky, kc = (kt - q) - ky (names that shadow nothing);
             # f32c adds the increment's low part: ky = dq_hi - (c - dq_lo)
             cterm = ("(C%(i)d[sb][o + CB] - incl%(i)d)" if self.compensated else "C%(i)d[sb][o + CB]")
@@ -505,14 +540,18 @@ extern "C" FormuraBatch* Formura_Batch_Create(int ninst, const int* n1, const in
   return b;
 }
 
-// state of one instance as NF fields of n1*n2 doubles (row-major); the
-// compensation terms start at zero
+// state of one instance as NF fields of n1*n2 doubles (row-major).  With Kahan
+// compensation a value x is stored as the pair (q, c) = (RN(x), RN(q - x)),
+// which carries x to about 2^-48 (q - c is the compensated value)
 extern "C" void Formura_Batch_Set_State(FormuraBatch* b, int i, const double* fields) {
   if (i < 0 || i >= b->ninst) fmr_rt::fatal("Formura_Batch_Set_State: bad instance");
   const Inst& I = b->inst[i];
   const long long plane = (long long)I.n1 * I.n2;
   std::vector<real_t> h((size_t)NSLOT * plane, (real_t)0);
-  for (long long e = 0; e < NF * plane; ++e) h[e] = (real_t)fields[e];
+  for (long long e = 0; e < NF * plane; ++e) {
+    h[e] = (real_t)fields[e];
+    if (NSLOT > NF) h[NF * plane + e] = (real_t)((double)h[e] - fields[e]);
+  }
   FMR_CHECK(hipMemcpyAsync(b->cur + I.base, h.data(), sizeof(real_t) * h.size(), hipMemcpyHostToDevice, b->stream));
   FMR_CHECK(hipStreamSynchronize(b->stream));
   b->steps_done[i] = 0;
@@ -522,10 +561,12 @@ extern "C" void Formura_Batch_Get_State(FormuraBatch* b, int i, double* fields) 
   if (i < 0 || i >= b->ninst) fmr_rt::fatal("Formura_Batch_Get_State: bad instance");
   const Inst& I = b->inst[i];
   const long long plane = (long long)I.n1 * I.n2;
-  std::vector<real_t> h((size_t)NF * plane);
+  std::vector<real_t> h((size_t)NSLOT * plane);
   FMR_CHECK(hipMemcpyAsync(h.data(), b->cur + I.base, sizeof(real_t) * h.size(), hipMemcpyDeviceToHost, b->stream));
   FMR_CHECK(hipStreamSynchronize(b->stream));
-  for (long long e = 0; e < NF * plane; ++e) fields[e] = (double)h[e];
+  // with Kahan compensation the value is q - c, formed in FP64
+  for (long long e = 0; e < NF * plane; ++e)
+    fields[e] = NSLOT > NF ? (double)h[e] - (double)h[NF * plane + e] : (double)h[e];
 }
 
 // drifting-frame offsets of instance i after the steps run so far

@@ -24,8 +24,10 @@ const char *Formura_Batch_Describe(void);
 /* ninst instances; instance i has grid n1[i] x n2[i] (multiples of the tile)
  * and parameters params[i*np .. i*np+np-1] (np = number of parameter names). */
 FormuraBatch *Formura_Batch_Create(int ninst, const int *n1, const int *n2, const double *params);
-/* Instance state as nfields row-major arrays of n1*n2 doubles (converted to the
- * library's precision; Kahan compensation terms reset to zero). */
+/* Instance state as nfields row-major arrays of n1*n2 doubles.  Converted to
+ * the library's precision; in the Kahan-compensated FP32 forms a value x is
+ * stored as the FP32 pair (q, c) = (RN(x), RN(q - x)) and read back as q - c
+ * (formed in FP64), the value the compensated accumulation carries. */
 void Formura_Batch_Set_State(FormuraBatch *b, int inst, const double *fields);
 void Formura_Batch_Get_State(FormuraBatch *b, int inst, double *fields);
 /* Frame offsets (offset_x, offset_y) of an instance after the steps so far. */

@@ -7,10 +7,14 @@ hipGraph-captured step loops).
   calls, each of which replays the captured graph.
 * f32k: FP32 with Kahan-compensated time accumulation, relative L2 error per
   field against the f64 batch below 1e-5.
-* f32c: f32k with the increment in float-float FP32 arithmetic; on three
-  instances of the 1024-instance ensemble where f32k's error after 5000 steps
-  exceeds 1e-5 or nearly does (a reaction near its steady balance), f32c's is
-  below 1e-5 and below f32k's.
+* the ensemble of the configuration (1024 instances, sides 64..1024, per-
+  instance F and k, 5000 steps): f32x -- the state as the FP32 pair (q, -c)
+  and the increment and update in float-float FP32 arithmetic -- is within
+  1e-5 relative L2 of the f64 batch on every instance; f32c (the increment in
+  float-float from the FP32 state, Kahan accumulation) and f32k (plain FP32
+  increment, Kahan accumulation) are not on all (counts printed); the f64
+  batch's smallest instances equal the front-end's graph evaluated on the CPU
+  with their own parameters, bit for bit, after the 5000 steps.
 """
 import numpy as np
 import pytest
@@ -97,26 +101,40 @@ def _rel_l2(x, y):
     return (num / den) ** 0.5
 
 
-def test_f32c_batch_error_against_f64():
+def test_ensemble_configuration_errors_and_f64_parity():
+    import mm_interp
+    from test_frontend import _mm
+    steps = 5000
     sizes, params, seeds = B.grayscott_ensemble(1024)
-    pick = [14, 101, 135]
-    sizes = [sizes[i] for i in pick]
-    params = params[pick]
-    seeds = [seeds[i] for i in pick]
-    ens = {n: _ens(n, sizes, params) for n in ("gs_batch_f64", "gs_batch_f32k", "gs_batch_f32c")}
-    for e in ens.values():
-        for i, (s, sd) in enumerate(zip(sizes, seeds)):
-            e.set_state(i, B.grayscott_init(s[0], sd))
-        e.step(5000)
-    for i in range(len(pick)):
-        x = ens["gs_batch_f64"].get_state(i)
-        ek = _rel_l2(x, ens["gs_batch_f32k"].get_state(i))
-        ec = _rel_l2(x, ens["gs_batch_f32c"].get_state(i))
-        print("instance %d (side %d): relative L2 error after 5000 steps f32k %.3g, f32c %.3g"
-              % (pick[i], sizes[i][0], ek, ec), flush=True)
-        assert ec <= 1e-5 and ec < ek, (pick[i], ek, ec)
-    for e in ens.values():
+    inits = [B.grayscott_init(s[0], sd) for s, sd in zip(sizes, seeds)]
+    errs = {}
+    ref = None
+    for name in ("gs_batch_f64", "gs_batch_f32x", "gs_batch_f32c", "gs_batch_f32k"):
+        print("ensemble of 1024 instances, %s, %d steps" % (name, steps), flush=True)
+        e = _ens(name, sizes, params)
+        for i, st in enumerate(inits):
+            e.set_state(i, st)
+        e.step(steps)
+        got = [e.get_state(i) for i in range(len(sizes))]
         e.close()
+        if ref is None:
+            ref = got
+            continue
+        errs[name] = np.array([_rel_l2(x, y) for x, y in zip(ref, got)])
+        print("  %s: %d of 1024 within 1e-5, worst %.3g, median %.3g" % (
+            name, int((errs[name] <= 1e-5).sum()), errs[name].max(), np.median(errs[name])), flush=True)
+    assert np.all(errs["gs_batch_f32x"] <= 1e-5), np.argsort(errs["gs_batch_f32x"])[-5:]
+    assert (errs["gs_batch_f32c"] <= 1e-5).sum() >= 900 and (errs["gs_batch_f32k"] <= 1e-5).sum() >= 800
+    # the f64 batch at the configuration: its two smallest instances against the
+    # front-end's graph with their own (F, k), bit for bit
+    mm = _mm("gs_64")
+    for i in sorted(range(len(sizes)), key=lambda i: sizes[i][0])[:2]:
+        print("  f64 instance %d (side %d) against the interpreter" % (i, sizes[i][0]), flush=True)
+        cpu = dict(inits[i])
+        for _ in range(steps):
+            cpu = mm_interp.step(mm.step, 1, cpu, {"f": params[i][0], "k": params[i][1]})
+        for f in ("u", "v"):
+            np.testing.assert_array_equal(ref[i][f], cpu[f])
 
 
 def _diffusion_mm():
