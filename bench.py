@@ -31,6 +31,22 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _issue_from_profile(kernel: str):
+    """VALU issue fraction of a kernel from the committed PMC summary
+    (profiles/valu_fp64_kernels.md: SQ_INSTS_VALU over the wave-instruction issue
+    peak during the kernel), or None."""
+    p = os.path.join(ROOT, "profiles", "valu_fp64_kernels.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 7 and cells[1] == "`%s`" % kernel:
+            m = re.match(r"([0-9.]+)\s*%", cells[7])
+            if m:
+                return float(m.group(1)) / 100.0
+    return None
+
+
 def _traffic_from_profile(kernel_tag: str):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
     p = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
@@ -78,6 +94,10 @@ def run_gpu(name: str, steps: int, warmup: int, bytes_per_lup: float, seed: int 
         "glups": glups, "ms": ms, "wall_s": wall, "steps": nsteps, "cells": cells,
         "launches": launches, "mean_launch_us": ms * 1e3 / launches,
         "bytes_per_launch": bytes_per_lup * cells * nsteps / launches,
+        # a launch must read the state once and write it once, however many
+        # steps it fuses: the minimum HBM bytes of one launch
+        "min_bytes_per_launch": bytes_per_lup * cells,
+        "steps_per_launch": nsteps / launches,
         "schedule": lib.layout.get("schedule"), "finite": finite,
     }
 
@@ -271,13 +291,62 @@ def run_per_instance_fp64(sizes, steps: int, probe_steps: int = 600):
             "us_per_step": {str(n): round(v * 1e3, 3) for n, v in sorted(per.items())}}
 
 
+def _roof(r, bytes_per_lup):
+    """GLUPS, the HBM fraction of the minimum bytes per launch (the state read
+    and written once), and the effective rate (bytes per update x updates)."""
+    min_gbs = r["min_bytes_per_launch"] / (r["mean_launch_us"] * 1e-6) / 1e9
+    return {"glups": r["glups"], "schedule": r["schedule"], "bytes_per_lup": bytes_per_lup,
+            "steps_per_launch": r["steps_per_launch"], "mean_launch_us": r["mean_launch_us"], "steps": r["steps"],
+            "min_traffic_gbs": min_gbs, "min_traffic_frac": min_gbs / HBM_PEAK_GBS,
+            "effective_gbs": r["glups"] * bytes_per_lup, "effective_frac": r["glups"] * bytes_per_lup / HBM_PEAK_GBS}
+
+
 def run_heat_roofline(name: str = "heat3d_512", steps: int = 200):
-    """Plain FP64 diffusion at full size: GLUPS and the effective HBM fraction
-    (16 B per cell update: read q, write q')."""
-    r = run_gpu(name, steps, 20, 16.0)
-    return {"glups": r["glups"], "schedule": r["schedule"], "bytes_per_lup": 16,
-            "effective_gbs": r["glups"] * 16.0, "effective_frac": r["glups"] * 16.0 / HBM_PEAK_GBS,
-            "mean_launch_us": r["mean_launch_us"], "steps": r["steps"]}
+    """Plain FP64 diffusion at full size: GLUPS, the HBM fraction of the minimum
+    traffic per launch, and the effective HBM fraction (16 B per cell update:
+    read q, write q')."""
+    return _roof(run_gpu(name, steps, 20, 16.0), 16.0)
+
+
+def run_sizes():
+    """The other plain-path configurations: Gray-Scott 8192^2 (1 GiB of state,
+    beyond the 256 MiB Infinity Cache) and the 3-D heat 64^3 of the CPU-runnable
+    configuration, without and with temporal blocking (4 steps per Forward)."""
+    return {"gs_8192": _roof(run_gpu("gs_8192", 400, 20, 32.0), 32.0),
+            "heat3d_64": _roof(run_gpu("heat3d_64", 100, 20, 16.0), 16.0),
+            "heat3d_64_tb": _roof(run_gpu("heat3d_64_tb", 100, 20, 16.0), 16.0)}
+
+
+def run_contract():
+    """The opt-in contract form (FMA contraction, reciprocal division by
+    constants) against the plain FP64 path from the same seeded state: GLUPS and
+    the relative max-norm error per field at the configurations' steps."""
+    import numpy as np
+    from formura_amd import inputs
+    from formura_amd.runtime import open_program
+    out = {}
+    for plain, fast, steps in (("gs_2048", "gs_2048_contract", 10000), ("euler3d_256", "euler3d_256_contract", 200)):
+        res = {}
+        for name in (plain, fast):
+            lib = open_program(name)
+            lib.init()
+            if "u" in lib.fields:
+                lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), 7))
+            else:
+                lib.set_state(inputs.euler_state(1, lib.grid))
+            lib.set_resident(True)
+            lib.upload()
+            ms = lib.forward_steps(steps)
+            lib.download()
+            res[name] = ({f: lib.field(f).copy() for f in lib.fields}, int(np.prod(lib.grid)) * steps / (ms * 1e-3) / 1e9)
+            lib.set_resident(False)
+            lib.finalize()
+        a, b = res[plain][0], res[fast][0]
+        out[fast] = {"steps": steps, "glups": res[fast][1], "plain_glups": res[plain][1],
+                     "speedup": res[fast][1] / res[plain][1],
+                     "rel_maxnorm_err_vs_fp64": {f: float(np.max(np.abs(a[f] - b[f])) / np.max(np.abs(a[f])))
+                                                 for f in a}}
+    return out
 
 
 def run_fp32_heat(steps: int = 200, seed: int = 23):
@@ -461,8 +530,43 @@ def run_adjoint(nsteps: int = 2000, snapshots: int = 32, directions: int = 16, e
     return out
 
 
-def cpu_baseline(name: str, target_s: float = 10.0):
-    """Oracle (fast build) on the host cores, timed on a bounded number of steps."""
+def host_cores():
+    """(physical cores of the host per lscpu, CPUs this process may run on, the
+    OpenMP threads a parallel region of the oracle runs with)."""
+    import ctypes
+    import subprocess
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=30).stdout
+        phys = len({ln for ln in out.splitlines() if ln and not ln.startswith("#")})
+    except Exception:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except Exception:
+        avail = os.cpu_count() or 1
+    try:  # the OpenMP runtime the oracle libraries link (already initialised by this process's imports)
+        threads = int(ctypes.CDLL("libgomp.so.1").omp_get_max_threads())
+    except Exception:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or avail
+    return phys, avail, threads
+
+
+# CPU baseline configurations: name -> (oracle library, seed function name, steps per timed sample)
+CPU_CONFIGS = {
+    "gs_2048": ("gs", 200),
+    "heat3d_64": ("heat", 100),
+    "heat3d_64_tb": ("heat", 25),       # 25 Forwards = 100 steps (4 per Forward)
+    "heat3d_256": ("heat", 4),
+    "euler3d_256": ("euler", 1),
+}
+
+
+def cpu_time(name: str, samples: int = 5, min_s: float = 1.0, threads: int = 1):
+    """The oracle's fast build (the reference Makefile flags, OpenMP) on the host
+    cores: ``samples`` timed samples of whole Formura_Forward calls (each at least
+    ``min_s`` seconds) after one warm-up call; returns the median GLUPS."""
+    import statistics
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import build_oracle
@@ -478,21 +582,56 @@ def cpu_baseline(name: str, target_s: float = 10.0):
         build_oracle.build(name, "fast")
     lib = FormuraLib(path, lib_description(mm.om, name, ""))
     lib.init()
-    lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), 7))
+    kind, chunk = CPU_CONFIGS[name]
+    if kind == "gs":
+        lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), 7))
+    elif kind == "euler":
+        lib.set_state(inputs.euler_state(1, lib.grid))
+    else:
+        lib.set_state(inputs.heat_field(7, lib.grid))
+    spf = lib.layout["steps_per_forward"]
     lib.forward(1)  # warm-up
     cells = int(np.prod(lib.grid))
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        lib.forward(1)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= target_s or n >= 100000:
-            break
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": cells * n / el / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
-            "sample": "%d steps of the %s workload (%.1f s), oracle C built with -O3 -ffast-math "
-                      "-mavx2 -mfma -fopenmp, OMP_NUM_THREADS=%d" % (n, name, el, threads)}
+    rates, total = [], 0.0
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            lib.forward(chunk)
+            n += chunk
+            el = time.perf_counter() - t0
+            if el >= min_s:
+                break
+        rates.append(cells * n * spf / el / 1e9)
+        total += el
+    return {"glups": statistics.median(rates), "samples": [round(r, 4) for r in rates], "seconds": round(total, 2),
+            "steps_per_forward": spf, "blocking": "temporal" if spf > 1 else "none", "threads": threads}
+
+
+def cpu_baseline(name: str, threads: int, samples: int = 5, min_s: float = 2.0):
+    """The headline workload on the host cores (see ``cpu_time``)."""
+    phys, avail, _ = host_cores()
+    r = cpu_time(name, samples, min_s, threads)
+    return {"value": r["glups"], "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": "median of %d samples of >= %.0f s of whole Formura_Forward calls (%.1f s in all) of the %s "
+                      "workload; oracle C (transcription of the reference's generated C) built with -O3 "
+                      "-ffast-math -mavx2 -mfma -fopenmp, %d OpenMP threads (OMP_PROC_BIND=%s); host: %s "
+                      "physical cores (lscpu), %d CPUs available to this process"
+                      % (samples, min_s, r["seconds"], name, threads, os.environ.get("OMP_PROC_BIND", "unset"),
+                         phys, avail),
+            "samples": r["samples"]}
+
+
+def cpu_baselines(threads: int):
+    """CPU rates of the other configurations (C1 64^3 without and with temporal
+    blocking, heat 256^3, C3 Euler 256^3), each the median of 5 samples."""
+    out = {}
+    for name in ("heat3d_64", "heat3d_64_tb", "heat3d_256", "euler3d_256"):
+        try:
+            out[name] = cpu_time(name, 5, 1.0, threads)
+        except Exception as e:
+            out[name] = {"error": str(e)[:200]}
+    return out
 
 
 def run_samples(steps: int = 50):
@@ -554,19 +693,29 @@ def run_adjoint_probe(nsteps: int = 200, snapshots: int = 8):
     return {"steps": nsteps, "snapshots": snapshots, "dL_dF": float(p.grad[0]), "stats": dict(s.last_stats)}
 
 
+# the legs after the headline, in order: (key in the JSON line, leg name, function)
+LEGS = (("grayscott2048_fp32", "fp32", run_fp32_single),
+        ("diffusion2d_2048_fp32", "fp32", run_fp32_diffusion),
+        ("heat3d_512_fp32", "fp32", run_fp32_heat),
+        ("heat3d_512", "heat", run_heat_roofline), ("sizes", "sizes", run_sizes),
+        ("euler3d_256", "euler", run_euler), ("contract", "contract", run_contract),
+        ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU time of the headline's CPU baseline (5 samples of a fifth of it)")
     ap.add_argument("--ensemble", type=int, default=1024,
                     help="instances of the batched FP32 Gray-Scott ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the heat 512^3, Euler 256^3 and adjoint legs")
-    ap.add_argument("--legs", default="plain,fp32,ensemble,heat,euler,adjoint,samples,cpu",
+    ap.add_argument("--legs", default="plain,fp32,ensemble,heat,sizes,euler,contract,adjoint,samples,cpu",
                     help="comma-separated legs to run (for profiling one kernel family): plain (Gray-Scott "
                          "2048^2, the headline value), fp32 (the same in the FP32 forms, error vs FP64), "
                          "ensemble, heat, euler, adjoint, samples, cpu; "
@@ -598,29 +747,30 @@ def main():
         sys.exit(2)
     legs = set(args.legs.split(","))
     if args.no_extras:
-        legs -= {"heat", "euler", "adjoint", "samples"}
+        legs -= {"heat", "sizes", "euler", "contract", "adjoint", "samples"}
     if args.no_cpu:
         legs.discard("cpu")
     if args.ensemble <= 0:
         legs.discard("ensemble")
     if "plain" not in legs:
         line = {"legs": sorted(legs)}
-        for key, leg, fn in (("grayscott2048_fp32", "fp32", run_fp32_single),
-                             ("diffusion2d_2048_fp32", "fp32", run_fp32_diffusion),
-                             ("heat3d_512_fp32", "fp32", run_fp32_heat),
-                             ("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
-                             ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples),
-                             ("adjoint_probe", "adjoint_probe", run_adjoint_probe)):
+        for key, leg, fn in LEGS + (("adjoint_probe", "adjoint_probe", run_adjoint_probe),):
             if leg in legs:
                 line[key] = fn()
         if "ensemble" in legs:
             line["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)
+        if "cpu" in legs:
+            line["cpu_baselines"] = cpu_baselines(host_cores()[2])
         print(json.dumps(line), flush=True)
         return
     name = "gs_2048"
     r = run_gpu(name, args.steps, args.warmup, 32.0)
-    achieved = r["bytes_per_launch"] / (r["mean_launch_us"] * 1e-6) / 1e9
+    # roofline of the dominant kernel: the minimum HBM bytes of one launch (the
+    # state read once and written once, 32 B per cell, whatever the number of
+    # fused steps) over its mean duration
+    achieved = r["min_bytes_per_launch"] / (r["mean_launch_us"] * 1e-6) / 1e9
     traffic = _traffic_from_profile(name)
+    issue = _issue_from_profile("k_step_tile<4>")
     line = {
         "metric": "GLUPS & % HBM roofline, FP32/batched vs plain FP64; adjoint/forward ratio",
         "value": round(r["glups"], 3),
@@ -639,7 +789,11 @@ def main():
                    "grid": [2048, 2048], "fields": 2, "schedule": r["schedule"],
                    "bytes_per_lup": 32, "resident": True},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_step_tile<%d>" % round(r["steps_per_launch"]),
+                     "bytes_model": "state read once and written once per launch: 32 B x cells",
+                     "effective_gbs": round(r["glups"] * 32.0, 1),
+                     "valu_issue_frac": issue},
     }
     if not r["finite"]:
         line["error"] = "non-finite state after the run"
@@ -656,21 +810,20 @@ def main():
             line["ensemble"] = ens
         except Exception as e:
             line["ensemble"] = {"error": str(e)[:300]}
-    for key, leg, fn in (("grayscott2048_fp32", "fp32", run_fp32_single),
-                         ("diffusion2d_2048_fp32", "fp32", run_fp32_diffusion),
-                         ("heat3d_512_fp32", "fp32", run_fp32_heat),
-                         ("heat3d_512", "heat", run_heat_roofline), ("euler3d_256", "euler", run_euler),
-                         ("adjoint", "adjoint", run_adjoint), ("samples", "samples", run_samples)):
+    for key, leg, fn in LEGS:
         if leg in legs:
             try:
                 line[key] = fn()
             except Exception as e:
                 line[key] = {"error": str(e)[:300]}
     if "cpu" in legs and int(os.environ.get("RANK", "0")) == 0:
+        threads = host_cores()[2]
         try:
-            line["cpu_baseline"] = cpu_baseline(name, args.cpu_seconds)
+            line["cpu_baseline"] = cpu_baseline(name, threads, min_s=args.cpu_seconds / 5)
+            line["cpu_baseline"]["gpu_vs_cpu"] = r["glups"] / line["cpu_baseline"]["value"]
         except Exception as e:  # the GPU number stands without it
             line["cpu_baseline"] = {"error": str(e)[:300]}
+        line["cpu_baselines"] = cpu_baselines(threads)
     print(json.dumps(line), flush=True)
 
 
