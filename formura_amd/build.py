@@ -226,6 +226,9 @@ ADJOINTS: Dict[str, tuple] = {
     "gs_1024_adj_f32": ("grayscott2d.fmr", "gs_1024.yaml", ["f", "k", "ru", "rv"], "float"),
     "heat3d_16_adj": ("heat3d.fmr", "heat3d_16.yaml", [], "double"),
     "heat3d_256_adj": ("heat3d.fmr", "heat3d_256.yaml", [], "double"),
+    # the Euler HLLC step: the adjoint of the flux Jacobian (wave speeds, star
+    # states and the flux selection differentiated through their branches)
+    "euler3d_16_adj": ("euler3d.fmr", "euler3d_16.yaml", [], "double"),
 }
 
 

@@ -209,3 +209,91 @@ def test_fused_adjoint_steps_match_single_steps(name):
         torch.cuda.synchronize()
         assert torch.equal(out, a), k
         torch.testing.assert_close(pg2.sum(0), pg1.sum(0), rtol=1e-12, atol=0)
+
+
+def test_euler_hllc_adjoint_forward_bit_exact_and_central_differences():
+    """The HLLC step's adjoint (reverse mode through the wave speeds, the star
+    states and the flux selection, one work-item per point): its forward equals
+    the oracle bit for bit, and the gradient of a linear functional of the state
+    after 6 steps matches FP64 central differences (eps = 1e-6) along 8 seeded
+    directions to 1e-6."""
+    s = _stepper("euler3d_16_adj")
+    fields = s.desc["fields"]
+    st = inputs.euler_state(2, (16, 16, 16))
+    x0 = np.stack([st[f] for f in fields])
+    steps = 6
+    o = oracle_lib("euler3d_16", layout_of("euler3d_16"))
+    o.init()
+    o.set_state(st)
+    o.forward(steps)
+    none = torch.zeros(0, dtype=torch.float64)
+    y = A.run(s, torch.tensor(x0, device=DEV), none, steps, 0).cpu().numpy()
+    for i, f in enumerate(fields):
+        np.testing.assert_array_equal(y[i], o.field(f))
+    rng = np.random.default_rng(77)
+    w = torch.tensor(rng.standard_normal(x0.shape), device=DEV)
+    x = torch.tensor(x0, device=DEV, requires_grad=True)
+    (w * A.run(s, x, none, steps, 2)).sum().backward()
+    g = x.grad.cpu().numpy()
+
+    def L(xx):
+        with torch.no_grad():
+            return float((w * A.run(s, torch.tensor(xx, device=DEV), none, steps, 0)).sum())
+
+    eps = 1e-6
+    for _ in range(8):
+        d = rng.standard_normal(x0.shape) * np.abs(x0).mean(axis=(1, 2, 3), keepdims=True) * 0.01
+        fd = (L(x0 + eps * d) - L(x0 - eps * d)) / (2 * eps)
+        an = float((g * d).sum())
+        print("HLLC adjoint: directional derivative %.12g, central difference %.12g" % (an, fd), flush=True)
+        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_heat_256_adjoint_transpose_identity():
+    """The heat step is linear: after 100 steps at 256^3, <A^T lam, x> =
+    <lam, A x> to rounding, for seeded random x and lam."""
+    s = _stepper("heat3d_256_adj")
+    n, steps = 256, 100
+    none = torch.zeros(0, dtype=torch.float64)
+    xq = torch.tensor(inputs.uniform(31, (n, n, n))[None] - 0.5, device=DEV)
+    lam = torch.tensor(inputs.uniform(32, (n, n, n))[None] - 0.5, device=DEV)
+    ax = A.run(s, xq, none, steps, 0)
+    x = xq.clone().requires_grad_(True)
+    (lam * A.run(s, x, none, steps, 4)).sum().backward()
+    lhs = float((x.grad * xq).sum())
+    rhs = float((lam * ax).sum())
+    scale = float(torch.linalg.norm(lam) * torch.linalg.norm(ax))
+    print("heat 256^3: <A^T lam, x> = %.15g, <lam, A x> = %.15g" % (lhs, rhs), flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale
+
+
+def test_configuration_gradients_central_differences():
+    """The adjoint configuration: Gray-Scott 1024^2, 2000 steps, 32 snapshots,
+    loss 0.5 ||v_T - v_target||^2 with v_target from a seeded forward run at
+    (F, k) = (0.035, 0.065); gradients w.r.t. the initial u, v and F, k, Du, Dv
+    match FP64 central differences (eps = 1e-6) along 16 seeded directions to
+    1e-6."""
+    steps, snaps = 2000, 32
+    s = _stepper("gs_1024_adj")
+    st = B.grayscott_init(1024, 11)
+    x0 = np.stack([st["u"], st["v"]])
+    with torch.no_grad():
+        vt = A.run(s, torch.tensor(x0, device=DEV), torch.tensor([0.035, 0.065, 2e-5, 1e-5], dtype=torch.float64),
+                   steps, 0)[1].clone()
+    loss, gx, gp = _gpu_loss_and_grads(s, x0, THETA, vt, steps, snaps)
+
+    def L(x, th):
+        with torch.no_grad():
+            y = A.run(s, torch.tensor(x, device=DEV), torch.tensor(th, dtype=torch.float64), steps, 0)
+            return 0.5 * float(((y[1] - vt) ** 2).sum())
+
+    rng = np.random.default_rng(2024)
+    eps, worst = 1e-6, 0.0
+    for _ in range(16):
+        dx = rng.standard_normal(x0.shape)
+        dp = rng.standard_normal(4) * np.array(THETA)
+        fd = (L(x0 + eps * dx, np.array(THETA) + eps * dp) - L(x0 - eps * dx, np.array(THETA) - eps * dp)) / (2 * eps)
+        an = float((gx * dx).sum() + (gp * dp).sum())
+        worst = max(worst, abs(fd - an) / abs(an))
+    print("Gray-Scott 1024^2 x 2000 steps, 32 snapshots: worst relative FD error %.3g" % worst, flush=True)
+    assert worst <= 1e-6

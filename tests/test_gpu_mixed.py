@@ -137,3 +137,33 @@ def test_mixed_staged_matches_materialized_flat():
         for res in out:
             err = float(np.max(np.abs(res[f] - o.field(f))) / np.max(np.abs(o.field(f))))
             assert 0 < err <= 1e-5, (f, err)
+
+
+def test_mixed_configuration_256_200_steps():
+    """The configuration's mixed variant: Euler HLLC 256^3, 200 steps, the
+    seeded state, FP32 fluxes (recomputed in place, and staged in LDS) against
+    the FP64 library (itself bit-exact with the oracle): relative max-norm
+    error per field within 1e-5, the five totals conserved to 1e-13 of their
+    scale, no positivity violation."""
+    steps = 200
+    st = None
+    out = {}
+    for name in ("euler3d_256", "euler3d_256_mixed", "euler3d_256_mixed_staged"):
+        print("Euler 256^3 %s, %d steps" % (name, steps), flush=True)
+        h = _seeded(name, lambda h: inputs.euler_state(1, h.grid))
+        t0 = h.totals()
+        h.forward_steps(steps)
+        t1 = h.totals()
+        h.download()
+        scale = max(abs(v) for v in t0.values())
+        assert max(abs(t1[f] - t0[f]) for f in t0) <= 1e-13 * scale
+        assert h.positivity_violations() == 0
+        out[name] = {f: h.field(f).copy() for f in h.fields}
+        h.set_resident(False)
+        h.finalize()
+    a = out["euler3d_256"]
+    for name in ("euler3d_256_mixed", "euler3d_256_mixed_staged"):
+        for f in a:
+            err = float(np.max(np.abs(out[name][f] - a[f])) / np.max(np.abs(a[f])))
+            print("  %s %s: %.3g" % (name, f, err), flush=True)
+            assert 0 < err <= 1e-5, (name, f, err)
