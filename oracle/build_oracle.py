@@ -45,6 +45,11 @@ CONFIGS = {
     "euler3d_16": ("euler3d_ref.c", {"NX": 16}),
     "euler3d_32": ("euler3d_ref.c", {"NX": 32}),
     "euler3d_256": ("euler3d_ref.c", {"NX": 256}),
+    # the reference's sample/1d_hllc/hllc.fmr (MUSCL-Hancock HLLC) on one rank
+    "hllc1d_64": ("hllc1d_ref.c", {"NX": 64, "LX": 4.0}),
+    "hllc1d_400": ("hllc1d_ref.c", {"NX": 400, "LX": 4.0}),
+    "hllc1d_128": ("hllc1d_ref.c", {"NX": 128, "LX": 4.0}),
+    "hllc1d_256": ("hllc1d_ref.c", {"NX": 256, "LX": 4.0}),
     # the reference's shock tube (sample/3d_hllc: 400 cells over 4.0 along the
     # tube, dx = 0.01, dt = C_cfl*dx/max_signal_speed = (0.8*0.01)/4, :19-21)
     # along each axis in turn, 4 cells across

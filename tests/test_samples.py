@@ -231,3 +231,29 @@ def test_reference_configuration_against_exact_solution(name):
             assert (v == v[:1, :, :1]).all(), f
     for f, e in _exact_l1(meta, name, st).items():
         assert e < meta["l1_bound"], (f, e)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["hllc1d_64", "hllc1d_400"])
+def test_sample_library_matches_hand_transcription(name):
+    """The HIP library of sample/1d_hllc/hllc.fmr against oracle/hllc1d_ref.c,
+    the transcription of the C formura emits for it (placement derived by hand,
+    independent of this front-end): state and Navi equal bit for bit."""
+    import test_oracle_hllc1d as T
+    path = B.lib_path(name)
+    if not os.path.exists(path):
+        pytest.skip("%s is built only where the formura checkout is present" % name)
+    lib = load_lib(current_lib(name))
+    lib.init()
+    o = T.oracle(name)
+    meta, _ = fixture(name)
+    print("sample %s against the transcription: %d steps" % (name, meta["steps"]), flush=True)
+    if meta["seed"] is not None:
+        st = inputs.conserved_state(meta["seed"], tuple(meta["grid"]), lib.fields, meta["gamma"])
+        lib.set_state(st)
+        o.set_state(st)
+    lib.forward(meta["steps"])
+    o.forward(meta["steps"])
+    assert lib.navi_dict()["offset_x"] == o.navi_dict()["offset_x"]
+    for f in lib.fields:
+        np.testing.assert_array_equal(lib.field(f), o.field(f), err_msg=f)
