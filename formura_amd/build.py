@@ -81,6 +81,8 @@ PROGRAMS: Dict[str, tuple] = {
                           {"kind": "staged3d", "ty": 16, "tz": 16, "cx": 8, "threads": 512, "inline": "inner"}),
     "index2d_32_tb": ("index2d.fmr", "index2d_32_tb.yaml", {"kind": "tile2d", "tx": 16, "ty": 16, "k": 2}),
     "index2d_32_tb8": ("index2d.fmr", "index2d_32_tb.yaml", {"kind": "tile2d", "tx": 8, "ty": 32, "k": 2, "threads": 128}),
+    # index reads inside a manifest temporary (recomputed where read, with its nest's index)
+    "indextmp2d_32": ("indextmp2d.fmr", "indextmp2d_32.yaml", "auto"),
     "index3d_16_tb": ("index3d.fmr", "index3d_16_tb.yaml", {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 2}),
     "gs_64_tile_k8": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 8}),
     "gs_64_tile_k5_inplace": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 5, "inplace": 1}),

@@ -326,6 +326,10 @@ class Body:
                 rhs = tmp_load(m.inst[2], m.inst[1])
             elif k == "LoadIndex":
                 rhs = index(m.inst[1])
+                if len(m.inst) > 2 and m.inst[2]:
+                    # an index read of an inlined temporary: its nest's loop index
+                    # differs from this one's by a constant (manifest.inline_temporaries)
+                    rhs = "(%s%+d)" % (rhs, m.inst[2])
             elif k == "Uniop":
                 op = m.inst[1]
                 if op.startswith("external-call/"):

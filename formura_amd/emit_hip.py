@@ -37,6 +37,19 @@ def _idx_expr(dim, names):
     return e
 
 
+def _temps_read_index(g) -> bool:
+    return any(n.micro[-1].inst[0] != "Store" and any(m.inst[0] == "LoadIndex" for m in n.micro)
+               for n in g.values())
+
+
+def _index_inlined(g):
+    """A hook graph with its temporaries inlined when one of them reads an index."""
+    if not _temps_read_index(g):
+        return g
+    from .manifest import inline_temporaries
+    return inline_temporaries(g)
+
+
 class LibEmitter:
     def __init__(self, mm, name: str, schedule: str = "auto", options=None):
         self.mm = mm
@@ -77,6 +90,12 @@ class LibEmitter:
             # temporaries stay materialized: precision decided over the whole graph
             self._prec, storage = mixed_precision_graph(mm.step)
         step_mm = mm.step
+        if _temps_read_index(mm.step) and self.cfg.blocking is not None:
+            raise EmitError("index access inside a manifest temporary is supported without temporal blocking only")
+        if _temps_read_index(mm.step) and not self.inlined:
+            # a temporary's index reads take its own nest's loop index, which a
+            # periodic materialization cannot hold: recompute it where it is read
+            self.inlined, self.inline_inner = True, False
         if self.inline_inner:
             from .manifest import inline_temporaries, inner_temporaries
             inner = inner_temporaries(mm.step)
@@ -100,8 +119,10 @@ class LibEmitter:
                 if (orig_rng[k].lower, orig_rng[k].upper) != (new_rng[k].lower, new_rng[k].upper):
                     raise EmitError("inlining changed the stencil range of %s" % step_mm[k].name)
         self.step = GraphPlan(step_mm, self.cfg.sleeve, self.dim, "step", storage=storage)
-        self.filter = GraphPlan(mm.filter, self.cfg.filter_sleeve, self.dim, "filter") if mm.filter else None
-        self.first = GraphPlan(mm.first_step, self.cfg.sleeve0, self.dim, "first_step") if mm.first_step else None
+        self.filter = (GraphPlan(_index_inlined(mm.filter), self.cfg.filter_sleeve, self.dim, "filter")
+                       if mm.filter else None)
+        self.first = (GraphPlan(_index_inlined(mm.first_step), self.cfg.sleeve0, self.dim, "first_step")
+                      if mm.first_step else None)
         # index reads in a temporally blocked step: every sub-step of a Forward
         # reads i + offset + block_offset with the Navi of the Forward's start
         # (Templates.hs:373,481), which only a schedule that runs one Forward

@@ -198,6 +198,12 @@ def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = 
     of them stay ``LoadCursor`` (their cursor shifted like any other), and the
     other temporaries are inlined into them as into the Store nodes.
 
+    Index reads keep the reference's values: a nest evaluates ``LoadIndex k``
+    as its own loop index plus the offsets (Templates.hs:481), and a consumer
+    at index i reads temporary X at X's index i + c + off(consumer) - off(X)
+    (Templates.hs:474), so an index read of X inlined through an accumulated
+    cursor c reads i + c_k + off(root) - off(X): ``("LoadIndex", k, delta)``.
+
     ``prec`` (node -> micro -> C type, from emit.mixed_precision on the nodes
     before inlining) is carried into the copies and is part of the sharing key,
     so a value keeps the precision of the expression it came from wherever it
@@ -205,6 +211,7 @@ def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = 
     the same FP32 computation as the one cell i-1 does as its right face.
     """
     out = {}
+    rng = calc_range(mm)
     for k in sorted(mm):
         node = mm[k]
         if node.micro[-1].inst[0] != "Store" and k not in keep:
@@ -233,9 +240,13 @@ def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = 
             elif k0 == "LoadCursor":
                 tgt = mm[m.inst[2]]
                 r = copy(tgt, _add(m.inst[1], shift), len(tgt.micro) - 1, {})
-            elif k0 == "Imm" or k0 == "LoadIndex":
+            elif k0 == "LoadIndex":
+                ax = m.inst[1]
+                delta = (m.inst[2] if len(m.inst) > 2 else 0) + shift[ax] + rng[k].offset - rng[src.omid].offset
+                r = emit_(("LoadIndex", ax, delta) if delta else ("LoadIndex", ax), m.typ)
+            elif k0 == "Imm":
                 r = emit_(m.inst, m.typ)
-                if k0 == "Imm" and m.src:
+                if m.src:
                     new[r].src = m.src
             elif k0 == "Uniop":
                 r = emit_(("Uniop", m.inst[1], copy(src, shift, m.inst[2], cache)), m.typ)

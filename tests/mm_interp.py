@@ -32,9 +32,11 @@ def _eval_node(node, fields, temps, base, params=None, shape=None, offsets=None)
         elif k == "LoadCursor":
             v = _shift(temps[m.inst[2]], [c + b for c, b in zip(m.inst[1], base)])
         elif k == "LoadIndex":
-            # i_k + offset_k (Templates.hs:481, block offset 0), as a double
+            # i_k + offset_k (Templates.hs:481, block offset 0), as a double; an
+            # inlined temporary's index read carries its nest's constant delta
             ax = m.inst[1]
-            ix = np.arange(shape[ax], dtype=np.float64) + float((offsets or [0] * len(shape))[ax])
+            ix = (np.arange(shape[ax], dtype=np.float64) + float((offsets or [0] * len(shape))[ax])
+                  + float(m.inst[2] if len(m.inst) > 2 else 0))
             v = ix.reshape([-1 if a == ax else 1 for a in range(len(shape))])
         elif k == "Uniop":
             a = vals[m.inst[2]]
@@ -85,6 +87,71 @@ def step(mm_graph, sleeve, fields, params=None, offsets=None):
             out[node.micro[-1].inst[1]] = np.array(v)
         else:
             temps[k] = np.array(v)
+    return out
+
+
+def step_nests(mm_graph, sleeve, fields, offsets=None, params=None):
+    """One step evaluated literally as the reference's generated C does it
+    (Templates.hs:336-346, 417-487): the state copied into buff with the
+    low-side halo (buff[j] = data[(j - 2s) mod N]); every manifest node, in
+    ascending id, a loop nest over its own extent N + 2s - size(range); a
+    state load at cursor d reads buff[i + d + off(range)], a temporary X at
+    X[i + d + off(range) - off(range X)] (X held over its own extent, not
+    periodically), an index read is the nest's loop index i + offset; Store
+    nodes write output i."""
+    rng = calc_range(mm_graph)
+    shape = next(iter(fields.values())).shape
+    dim = len(shape)
+    offsets = offsets or [0] * dim
+    s2 = 2 * sleeve
+    buff = {f: np.pad(a, [(s2, 0)] * dim, mode="wrap") for f, a in fields.items()}
+    temps, out = {}, {}
+    ext = {"sqrt": np.sqrt, "exp": np.exp, "fabs": np.fabs, "sin": np.sin, "cos": np.cos}
+    for k in sorted(mm_graph):
+        node = mm_graph[k]
+        r = rng[k]
+        E = [n + s2 - r.size for n in shape]
+        vals = []
+        for m in node.micro:
+            kk = m.inst[0]
+            if m.src in (params or {}) and kk not in ("LoadCursorStatic", "LoadCursor", "LoadIndex", "Store"):
+                v = float(params[m.src])  # a literal bound to another value
+            elif kk == "Imm":
+                v = float(m.inst[1])
+            elif kk == "LoadCursorStatic":
+                o = [c + r.offset for c in m.inst[1]]
+                v = buff[m.inst[2]][tuple(slice(a, a + e) for a, e in zip(o, E))]
+            elif kk == "LoadCursor":
+                o = [c + r.offset - rng[m.inst[2]].offset for c in m.inst[1]]
+                v = temps[m.inst[2]][tuple(slice(a, a + e) for a, e in zip(o, E))]
+            elif kk == "LoadIndex":
+                ax = m.inst[1]
+                ix = np.arange(E[ax], dtype=np.float64) + float(offsets[ax])
+                v = ix.reshape([-1 if a == ax else 1 for a in range(dim)])
+            elif kk == "Uniop":
+                a = vals[m.inst[2]]
+                op = m.inst[1]
+                v = -a if op == "-" else (a if op == "+" else ext[op.split("/", 1)[1]](a))
+            elif kk == "Binop":
+                a, b = vals[m.inst[2]], vals[m.inst[3]]
+                with np.errstate(all="ignore"):
+                    v = {"+": lambda: a + b, "-": lambda: a - b, "*": lambda: a * b, "/": lambda: np.divide(a, b),
+                         "<": lambda: a < b, ">": lambda: a > b, "<=": lambda: a <= b, ">=": lambda: a >= b,
+                         "==": lambda: a == b, "!=": lambda: a != b, "&&": lambda: np.logical_and(a, b),
+                         "||": lambda: np.logical_or(a, b)}[m.inst[1]]()
+            elif kk == "Triop":
+                v = np.where(vals[m.inst[2]], vals[m.inst[3]], vals[m.inst[4]])
+            elif kk == "Store":
+                v = vals[m.inst[2]]
+            else:
+                raise NotImplementedError(kk)
+            vals.append(v)
+        v = np.array(np.broadcast_to(np.asarray(vals[-1], dtype=np.float64), E))
+        if node.micro[-1].inst[0] == "Store":
+            assert list(E) == list(shape)
+            out[node.micro[-1].inst[1]] = v
+        else:
+            temps[k] = v
     return out
 
 
