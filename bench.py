@@ -637,15 +637,17 @@ def cpu_baselines(threads: int):
 def run_samples(steps: int = 50):
     """The reference's sample programs at 128^3 (3-D HLLD MHD with GLM cleaning,
     3-D HLLC with MUSCL-Hancock), compiled from their .fmr by this front-end
-    with the default schedule: GLUPS with the state resident in HBM, and the
-    FP64 totals' relative drift (every field but the damped psi)."""
+    with the default schedule (cheap temporaries recomputed where read), with
+    every temporary materialized (``_flat``), and in the mixed form: GLUPS with
+    the state resident in HBM, and the FP64 totals' relative drift (every field
+    but the damped psi)."""
     import numpy as np
     from formura_amd import build as B, inputs
     from formura_amd.runtime import open_program
     out = {"steps": steps}
     finals = {}
-    for name, gamma in (("mhd3d_128", 5.0 / 3.0), ("mhd3d_128_mixed", 5.0 / 3.0),
-                        ("hllc3d_128", 7.0 / 5.0), ("hllc3d_128_mixed", 7.0 / 5.0)):
+    for name, gamma in (("mhd3d_128", 5.0 / 3.0), ("mhd3d_128_mixed", 5.0 / 3.0), ("mhd3d_128_flat", 5.0 / 3.0),
+                        ("hllc3d_128", 7.0 / 5.0), ("hllc3d_128_mixed", 7.0 / 5.0), ("hllc3d_128_flat", 7.0 / 5.0)):
         if not os.path.exists(B.lib_path(name)):
             out[name] = {"skipped": "built only where the formura checkout is present"}
             continue
@@ -669,6 +671,10 @@ def run_samples(steps: int = 50):
         lib.set_resident(False)
         lib.finalize()
     for base in ("mhd3d_128", "hllc3d_128"):
+        a, c = finals.get(base), finals.get(base + "_flat")
+        if a and c:
+            out[base]["identical_to_materialized_flat"] = all(bool(np.array_equal(a[f], c[f])) for f in a)
+            out[base]["speedup_vs_materialized_flat"] = out[base]["glups"] / out[base + "_flat"]["glups"]
         a, b = finals.get(base), finals.get(base + "_mixed")
         if a and b:
             out[base + "_mixed"]["rel_maxnorm_err_vs_fp64"] = {

@@ -139,10 +139,15 @@ SAMPLES: Dict[str, tuple] = {
     "mhd3d_128": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_128.yaml", "auto"),
     "hllc3d_128": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_128.yaml", "auto"),
     # mixed form: FP32 fluxes, FP64 wave speeds, reconstructions, positivity checks and update
-    "mhd3d_8_mixed": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_8.yaml", {"kind": "flat", "mixed": 1}),
-    "hllc3d_8_mixed": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_8.yaml", {"kind": "flat", "mixed": 1}),
-    "mhd3d_128_mixed": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_128.yaml", {"kind": "flat", "mixed": 1}),
-    "hllc3d_128_mixed": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_128.yaml", {"kind": "flat", "mixed": 1}),
+    "mhd3d_8_mixed": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_8.yaml", {"kind": "flat", "mixed": 1, "inline": "cheap"}),
+    "hllc3d_8_mixed": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_8.yaml", {"kind": "flat", "mixed": 1, "inline": "cheap"}),
+    "mhd3d_128_mixed": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_128.yaml", {"kind": "flat", "mixed": 1, "inline": "cheap"}),
+    "hllc3d_128_mixed": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_128.yaml", {"kind": "flat", "mixed": 1, "inline": "cheap"}),
+    # every temporary materialized (the default recomputes the cheap ones where read)
+    "hllc3d_8_flat": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_8.yaml", {"kind": "flat"}),
+    "mhd3d_8_flat": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_8.yaml", {"kind": "flat"}),
+    "hllc3d_128_flat": ("sample/3d_hllc/3d_hllc.fmr", "hllc3d_128.yaml", {"kind": "flat"}),
+    "mhd3d_128_flat": ("sample/mhd_3d_hlld/mhd_3d_hlld.fmr", "mhd3d_128.yaml", {"kind": "flat"}),
 }
 
 

@@ -68,11 +68,20 @@ class LibEmitter:
         if schedule == "auto" and spec is None:
             from .emit_hip import _tuned_spec
             spec = _tuned_spec(name)
+        if schedule == "auto" and spec is None:
+            # untuned programs with cheap manifest temporaries (limited
+            # reconstructions, single-cursor wave speeds): recompute those where
+            # read, one kernel per remaining level (the reference's MUSCL-Hancock
+            # HLLC at 128^3: 2.3 -> 5.9 GLUPS, HLLD MHD 1.3 -> 2.8, bit-identical)
+            from .manifest import cheap_temporaries
+            if cheap_temporaries(mm.step):
+                spec = {"kind": "flat", "inline": "cheap"}
         self.inlined = bool(isinstance(spec, dict) and spec.get("inline"))
         # "inline": "inner" recomputes only the temporaries no Store node reads
         # (wave speeds read by the flux temporaries) inside their readers;
         # the temporaries the update reads stay materialized
-        self.inline_inner = self.inlined and spec.get("inline") == "inner"
+        self.inline_inner = self.inlined and spec.get("inline") in ("inner", "cheap")
+        self.inline_policy = spec.get("inline") if self.inline_inner else None
         if self.inline_inner:
             self.inlined = False
         # mixed precision: FP32 arithmetic except branch decisions, square-root
@@ -95,10 +104,10 @@ class LibEmitter:
         if _temps_read_index(mm.step) and not self.inlined:
             # a temporary's index reads take its own nest's loop index, which a
             # periodic materialization cannot hold: recompute it where it is read
-            self.inlined, self.inline_inner = True, False
+            self.inlined, self.inline_inner, self.inline_policy = True, False, None
         if self.inline_inner:
-            from .manifest import inline_temporaries, inner_temporaries
-            inner = inner_temporaries(mm.step)
+            from .manifest import cheap_temporaries, inline_temporaries, inner_temporaries
+            inner = inner_temporaries(mm.step) if self.inline_policy == "inner" else cheap_temporaries(mm.step)
             keep = frozenset(k for k in mm.step if mm.step[k].micro[-1].inst[0] != "Store" and k not in inner)
             orig_rng = calc_range(mm.step)
             step_mm = inline_temporaries(mm.step, keep=keep)
@@ -1567,8 +1576,10 @@ def candidate_specs(em) -> List[dict]:
     out = [{"kind": "flat"}]
     if em.mm.step and any(n.micro[-1].inst[0] != "Store" for n in em.mm.step.values()):
         # programs with manifest temporaries: also try recomputing them where read
-        from .manifest import inline_temporaries
+        from .manifest import cheap_temporaries, inline_temporaries
         out.append({"kind": "flat", "inline": 1})
+        if cheap_temporaries(em.mm.step):
+            out.append({"kind": "flat", "inline": "cheap"})
         if em.dim == 3:
             bpc = sum(8 if ct == "double" else 4 for _, ct in em.fields)
             s = em.cfg.sleeve

@@ -185,6 +185,30 @@ def inner_temporaries(mm: Dict[int, MMNode]) -> set:
     return {k for k, n in mm.items() if n.micro[-1].inst[0] != "Store" and k not in read_by_store}
 
 
+def cheap_temporaries(mm: Dict[int, MMNode]) -> set:
+    """Temporaries worth recomputing where read: those no Store node reads that
+    either evaluate no division, square root or other external call (limited
+    reconstructions: a few comparisons and products) or that every reader reads
+    at one and the same cursor (wave speeds of one face: a merged kernel
+    evaluates them once)."""
+    read_by_store, cursors = set(), {}
+    for node in mm.values():
+        for m in node.micro:
+            if m.inst[0] == "LoadCursor":
+                cursors.setdefault(m.inst[2], set()).add(tuple(m.inst[1]))
+                if node.micro[-1].inst[0] == "Store":
+                    read_by_store.add(m.inst[2])
+    out = set()
+    for k, n in mm.items():
+        if n.micro[-1].inst[0] == "Store" or k in read_by_store:
+            continue
+        costly = any((m.inst[0] == "Binop" and m.inst[1] in ("/", "**")) or
+                     (m.inst[0] == "Uniop" and m.inst[1].startswith("external-call/")) for m in n.micro)
+        if not costly or len(cursors.get(k, set())) <= 1:
+            out.add(k)
+    return out
+
+
 def inline_temporaries(mm: Dict[int, MMNode], prec: Dict[int, Dict[int, str]] = None,
                        keep=frozenset()) -> Dict[int, MMNode]:
     """Replace every ``LoadCursor(c, X)`` of the Store nodes by X's own micro DAG
