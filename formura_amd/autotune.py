@@ -135,7 +135,7 @@ def batch_candidates(precision: str):
     """Tile (tx, ty), steps per launch k and persistent grid size for a batch
     library; ty and tx divide every side of the ensemble (multiples of 32) and
     the LDS of a workgroup stays within 64 KiB."""
-    arrays, size = (4, 4) if precision in ("f32k", "f32c") else (2, 8)
+    arrays, size = (4, 4) if precision in ("f32k", "f32c", "f32x") else (2, 8)
     out = []
     for tx, ty in ((16, 32), (32, 32), (8, 32), (16, 16), (32, 16)):
         for k in (2, 3, 4, 5, 6, 8):
