@@ -8,6 +8,11 @@ from the program's init graph (u = 1, v = 0, central square of 205 cells with
 u = 0.5, v = 0.25) plus seeded 1 % noise on the square; 10000 steps by default.
 
 The state is resident in HBM for the timed region (Formura_HIP_Set_Resident).
+The GPU's clocks ramp up over about a second of load from idle and fall back
+within a fraction of a second of idling, so every timed region follows
+``--settle`` seconds (default 1) of untimed steps of the same kernel, after
+which the state is reset to the seeded one: the timed steps are the job's own
+steps from its initial state (then the --warmup steps), at settled clocks.
 ``value`` is GLUPS = cells x steps / s / 1e9.  ``roofline`` is measured on the
 dominant kernel: algorithmic bytes per launch (32 B per cell update: read u, v,
 write u', v' in FP64) over its mean launch duration from HIP events recorded on
@@ -29,6 +34,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+# seconds of untimed work of the kernel about to be timed (set from --settle)
+SETTLE_S = 1.0
+
+
+def settle(step, seconds=None):
+    """Call ``step()`` (untimed steps of the kernel about to be timed) until
+    ``seconds`` (default SETTLE_S) of wall time have passed; returns the calls."""
+    t0, n = time.perf_counter(), 0
+    while time.perf_counter() - t0 < (SETTLE_S if seconds is None else seconds):
+        step()
+        n += 1
+    return n
 
 
 def _issue_from_profile(kernel: str):
@@ -66,12 +84,18 @@ def run_gpu(name: str, steps: int, warmup: int, bytes_per_lup: float, seed: int 
     lib = open_program(name)
     lib.init()
     if "u" in lib.fields:
-        lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), seed))
+        st = inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), seed)
     else:
-        lib.set_state(inputs.heat_field(seed, lib.grid))
+        st = inputs.heat_field(seed, lib.grid)
+    lib.set_state(st)
     lib.set_resident(True)
     lib.upload()
     spf = lib.layout["steps_per_forward"]
+    t_settle = time.perf_counter()
+    settle(lambda: lib.forward_steps(max(1, 48 // spf)))
+    t_settle = time.perf_counter() - t_settle
+    lib.set_state(st)  # the job's steps from its initial state
+    lib.upload()
     if warmup:
         lib.forward_steps(max(1, warmup // spf))
     try:
@@ -98,7 +122,7 @@ def run_gpu(name: str, steps: int, warmup: int, bytes_per_lup: float, seed: int 
         # steps it fuses: the minimum HBM bytes of one launch
         "min_bytes_per_launch": bytes_per_lup * cells,
         "steps_per_launch": nsteps / launches,
-        "schedule": lib.layout.get("schedule"), "finite": finite,
+        "schedule": lib.layout.get("schedule"), "finite": finite, "settle_s": round(t_settle, 3),
     }
 
 
@@ -129,6 +153,7 @@ def run_ensemble(count: int, steps: int):
         for i, st in enumerate(inits):
             e.set_state(i, st)
         e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"])  # warm-up, captures the graph
+        settle(lambda: e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"]))
         for i, st in enumerate(inits):
             e.set_state(i, st)
         ms = e.step(steps)
@@ -202,6 +227,7 @@ def run_fp32_single(steps: int = 10000, seed: int = 7):
         e = B.Ensemble(name, [(2048, 2048)], params)
         e.set_state(0, st)
         e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"])  # warm-up, captures the graph
+        settle(lambda: e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"]))
         e.set_state(0, st)
         ms = e.step(steps)
         res[name] = e.get_state(0)
@@ -245,6 +271,7 @@ def run_fp32_diffusion(steps: int = 10000, seed: int = 7):
         e = B.Ensemble(name, [(n, n)], np.zeros((1, 1)))
         e.set_state(0, {"q": q0})
         e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"])  # warm-up, captures the graph
+        settle(lambda: e.step(e.desc["steps_per_launch"] * e.desc["graph_launches"]))
         e.set_state(0, {"q": q0})
         ms = e.step(steps)
         res[name] = e.get_state(0)["q"]
@@ -275,10 +302,15 @@ def run_per_instance_fp64(sizes, steps: int, probe_steps: int = 600):
     for n, c in sorted(counts.items()):
         lib = open_program(B.side_name(n))
         lib.init()
-        lib.set_state(inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), 7))
+        st = inputs.grayscott_perturb(lib.field("u").copy(), lib.field("v").copy(), 7)
+        lib.set_state(st)
         lib.set_resident(True)
         lib.upload()
         spf = lib.layout["steps_per_forward"]
+        # the sides are timed back to back: the first settles the clocks, the rest keep them
+        settle(lambda: lib.forward_steps(max(1, 48 // spf)), None if not per else 0.2)
+        lib.set_state(st)
+        lib.upload()
         lib.forward_steps(max(1, 30 // spf))
         k = max(1, probe_steps // spf)
         ms = lib.forward_steps(k)
@@ -336,6 +368,10 @@ def run_contract():
                 lib.set_state(inputs.euler_state(1, lib.grid))
             lib.set_resident(True)
             lib.upload()
+            st = lib.state()
+            settle(lambda: lib.forward_steps(8))
+            lib.set_state(st)
+            lib.upload()
             ms = lib.forward_steps(steps)
             lib.download()
             res[name] = ({f: lib.field(f).copy() for f in lib.fields}, int(np.prod(lib.grid)) * steps / (ms * 1e-3) / 1e9)
@@ -370,7 +406,7 @@ def run_fp32_heat(steps: int = 200, seed: int = 23):
         lib.set_state(st)
         lib.set_resident(True)
         lib.upload()
-        lib.forward_steps(20)  # warm-up
+        settle(lambda: lib.forward_steps(8))
         lib.set_state(st)
         lib.upload()
         ms = lib.forward_steps(steps)
@@ -404,6 +440,9 @@ def run_euler(steps: int = 200):
         st = st or inputs.euler_state(1, lib.grid)
         lib.set_state(st)
         lib.set_resident(True)
+        lib.upload()
+        settle(lambda: lib.forward_steps(4))
+        lib.set_state(st)
         lib.upload()
         t0 = lib.totals()
         ms = lib.forward_steps(steps)
@@ -455,6 +494,12 @@ def run_adjoint(nsteps: int = 2000, snapshots: int = 32, directions: int = 16, e
         with torch.no_grad():
             vt = A.run(s, xd, torch.tensor([0.035, 0.065, 2e-5, 1e-5], dtype=torch.float64), nsteps, 0)[1].clone()
         fwd, bwd = [], []
+
+        def fwd_only():
+            with torch.no_grad():
+                A.run(s, xd, torch.tensor(theta, dtype=torch.float64), 100, 0)
+            torch.cuda.synchronize()
+        settle(fwd_only)
         for rep in range(4):  # one warm-up, then the best of three timed repetitions
             x = xd.clone().requires_grad_(True)
             p = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
@@ -501,6 +546,12 @@ def run_adjoint(nsteps: int = 2000, snapshots: int = 32, directions: int = 16, e
     tgt = torch.tensor(inputs.uniform(6, (256, 256, 256)), device=dev)
     hsteps = 100
     x = torch.tensor(q0, device=dev, requires_grad=True)
+
+    def heat_fwd():
+        with torch.no_grad():
+            A.run(s, x.detach(), torch.zeros(0, dtype=torch.float64), 10, 0)
+        torch.cuda.synchronize()
+    settle(heat_fwd)
     torch.cuda.synchronize()
     e0 = ev()
     y = A.run(s, x, torch.zeros(0, dtype=torch.float64), hsteps, snapshots)
@@ -653,10 +704,13 @@ def run_samples(steps: int = 50):
             continue
         lib = open_program(name)
         lib.init()
-        lib.set_state(inputs.conserved_state(5, tuple(lib.grid), lib.fields, gamma))
+        st = inputs.conserved_state(5, tuple(lib.grid), lib.fields, gamma)
+        lib.set_state(st)
         lib.set_resident(True)
         lib.upload()
-        lib.forward_steps(2)
+        settle(lambda: lib.forward_steps(2))
+        lib.set_state(st)
+        lib.upload()
         t0 = lib.totals()
         ms = lib.forward_steps(steps)
         t1 = lib.totals()
@@ -713,6 +767,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--settle", type=float, default=1.0,
+                    help="seconds of untimed steps before each timed region (GPU clock ramp); the state is "
+                         "then reset to the seeded one")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU time of the headline's CPU baseline (5 samples of a fifth of it)")
@@ -730,6 +787,8 @@ def main():
                     help="only run a plain device-to-device copy of 256 MiB, 20 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    global SETTLE_S
+    SETTLE_S = args.settle
     if args.calibrate_copy:
         import torch
         a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
@@ -784,6 +843,7 @@ def main():
         "n_gpus": 1,
         "steps": r["steps"],
         "warmup": args.warmup,
+        "settle_s": r["settle_s"],
         "ms_per_step": r["ms"] / r["steps"],
         "higher_is_better": True,
         "scaling": "weak",

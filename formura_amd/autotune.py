@@ -34,7 +34,7 @@ from . import build as B
 
 
 def _tag(spec: dict) -> str:
-    return "_".join(str(spec[k]) for k in sorted(spec)) if spec else "default"
+    return "_".join("%s%s" % (k, spec[k]) for k in sorted(spec)) if spec else "default"
 
 
 def variant_dir(name: str) -> str:
@@ -82,14 +82,26 @@ def _seed_state(lib, seed):
     return _STATE_CACHE[key]
 
 
-def run_variants(name: str, out_dir: str, seconds: float = 0.3, seed: int = 5):
+def run_variants(name: str, out_dir: str, seconds: float = 0.3, seed: int = 5, settle: float = 0.8,
+                 job_steps: int = 0):
+    """Time every built variant of ``name`` on the same seeded state.
+
+    The GPU's clocks ramp up from idle over about a second of load and fall
+    back within a fraction of a second of idling, so each variant first runs
+    ``settle`` seconds of untimed steps.  With ``job_steps``, the state is then
+    reset to the seeded one and exactly ``job_steps`` steps are timed (a whole
+    job from its initial state: for Gray-Scott, the early steps whose
+    diffusion front carries dividends outside the fused division's range are
+    included); otherwise the timed run continues from the settled state and
+    lasts about ``seconds``.  Every variant's final state must be bit-identical
+    to the first variant's."""
     import numpy as np
-    from . import inputs
     from .runtime import FormuraLib
     root = variant_dir(name)
     dirs = sorted(d for d in os.listdir(root) if os.path.exists(os.path.join(root, d, "lib%s.so" % name)))
     results, ref_state, ref_tag = [], None, None
-    steps = None
+    steps = job_steps or None
+    settle_steps = 0
     for d in dirs:
         spec = json.load(open(os.path.join(root, d, "spec.json")))
         lib = FormuraLib(os.path.join(root, d, "lib%s.so" % name))
@@ -98,16 +110,22 @@ def run_variants(name: str, out_dir: str, seconds: float = 0.3, seed: int = 5):
         lib.set_resident(True)
         lib.upload()
         cells = int(np.prod(lib.grid))
-        if steps is None:  # size the run from the first variant: ~seconds of work
-            ms = lib.forward_steps(4)
-            per = max(ms / 4, 1e-3)
-            steps = int(max(24, min(20000, seconds * 1e3 / per))) // 24 * 24
-            lib.download()
-            lib.finalize()
-            lib.init()
+        if job_steps:
+            t0 = time.time()
+            while time.time() - t0 < settle:
+                lib.forward_steps(48)
             lib.set_state(_seed_state(lib, seed))
             lib.upload()
-        lib.forward_steps(min(steps, 48))  # warm-up
+        else:
+            if steps is None:  # size the run from the first variant: ~seconds of work
+                ms = lib.forward_steps(4)
+                per = max(ms / 4, 1e-3)
+                steps = int(max(24, min(10 ** 6, seconds * 1e3 / per))) // 24 * 24
+                # the same number of settling steps for every variant: final states stay comparable
+                settle_steps = int(max(24, min(10 ** 6, settle * 1e3 / per))) // 24 * 24
+                lib.set_state(_seed_state(lib, seed))
+                lib.upload()
+            lib.forward_steps(settle_steps)
         ms = lib.forward_steps(steps)
         lib.download()
         st = lib.state()
@@ -117,9 +135,11 @@ def run_variants(name: str, out_dir: str, seconds: float = 0.3, seed: int = 5):
         else:
             same = all(np.array_equal(st[f], ref_state[f]) for f in st)
         glups = cells * steps / (ms * 1e-3) / 1e9
-        results.append({"spec": spec, "tag": d, "glups": glups, "ms": ms, "steps": steps,
-                        "launches": lib.last_launches(), "identical_to_first": bool(same)})
+        results.append({"spec": spec, "tag": d, "glups": glups, "ms": ms, "steps": steps, "settle_s": settle,
+                        "from_initial_state": bool(job_steps), "launches": lib.last_launches(),
+                        "identical_to_first": bool(same)})
         print("%-40s %8.2f GLUPS  %s" % (d, glups, "ok" if same else "MISMATCH"), flush=True)
+        lib.set_resident(False)
         lib.finalize()
     good = [r for r in results if r["identical_to_first"]]
     best = max(good, key=lambda r: r["glups"]) if good else None
@@ -207,6 +227,10 @@ def main():
     ap.add_argument("names", nargs="+")
     ap.add_argument("--out", default="tuning_out")
     ap.add_argument("--seconds", type=float, default=0.3)
+    ap.add_argument("--settle", type=float, default=0.8,
+                    help="seconds of untimed steps before each variant's timed run (clock ramp)")
+    ap.add_argument("--job-steps", type=int, default=0,
+                    help="time exactly this many steps from the seeded state (after settling)")
     a = ap.parse_args()
     for n in a.names:
         batch = n in B.BATCHES
@@ -215,7 +239,7 @@ def main():
         elif batch:
             run_batch_variants(n, a.out)
         else:
-            run_variants(n, a.out, a.seconds)
+            run_variants(n, a.out, a.seconds, settle=a.settle, job_steps=a.job_steps)
 
 
 if __name__ == "__main__":

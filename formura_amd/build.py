@@ -90,6 +90,18 @@ PROGRAMS: Dict[str, tuple] = {
     "gs_64_tile_t128": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 3, "threads": 128}),
     "gs_64_tile_92k": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 64, "ty": 64, "k": 6, "inplace": 1, "threads": 1024}),
     "gs_2048_wave": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "wave2d", "rows": 32, "k": 3}),
+    # how a division by a constant meets dividends outside the fused sequence's
+    # range: the whole point again for such a wave (spec), or a per-lane branch
+    # in each division (guard); the default redoes the division for such a wave
+    "gs_64_tile_spec": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "div": "spec"}),
+    "gs_64_tile_guard": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "div": "guard"}),
+    # a level's work-items on a fixed column each; two rows per work-item
+    "gs_64_tile_colmap": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "colmap": 1}),
+    "gs_64_tile_k5_inplace_colmap": ("grayscott2d.fmr", "gs_64.yaml",
+                                     {"kind": "tile2d", "tx": 16, "ty": 32, "k": 5, "inplace": 1, "colmap": 1}),
+    "gs_64_tile_vpair": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "vpair": 1}),
+    "gs_72x120_tile_colmap": ("grayscott2d.fmr", "gs_72x120.yaml",
+                              {"kind": "tile2d", "tx": 24, "ty": 40, "k": 3, "colmap": 1}),
     # the opt-in contract form (FMA contraction, reciprocal division; not bit-exact)
     "gs_2048_contract": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "tile2d", "tx": 32, "ty": 32, "k": 4,
                                                               "threads": 512, "contract": 1}),

@@ -49,12 +49,15 @@ __device__ __forceinline__ int fmr_mod(int i) {
 //   Arithmetic, 2nd ed., Thm. 4.10): b*q1 - a is exact for a faithful q1, and
 //   y has relative error below 2^-53.
 // The theorem needs no underflow or overflow: with 2^-900 <= |a| <= 2^900 the
-// quotient and the remainders are normal.  Other a (tiny, huge, Inf, NaN) take
-// the division.  The remainder is formed as b*q - a and subtracted, so a zero
-// dividend keeps its sign.
-__host__ __device__ __forceinline__ double fmr_divc(double a, double b, double y) {
+// quotient and the remainders are normal (fmr_divc_in_range).  Other a (tiny,
+// huge, Inf, NaN) take the division.  The remainder is formed as b*q - a and
+// subtracted, so a zero dividend keeps its sign.
+__host__ __device__ __forceinline__ bool fmr_divc_in_range(double a) {
   const double aa = fabs(a);
-  if (__builtin_expect(!((aa >= 0x1p-900 && aa <= 0x1p+900) || aa == 0.0), 0)) return a / b;
+  return (aa >= 0x1p-900 && aa <= 0x1p+900) || aa == 0.0;
+}
+
+__host__ __device__ __forceinline__ double fmr_divc_fast(double a, double b, double y) {
   double q = a * y;
   double r = fma(b, q, -a);
   q = fma(-r, y, q);
@@ -62,19 +65,50 @@ __host__ __device__ __forceinline__ double fmr_divc(double a, double b, double y
   return fma(-r, y, q);
 }
 
-// The same with one correction, for divisors whose host-rounded reciprocal is
-// close enough: if |b*y - 1| <= 2^-54 then q0 = RN(a*y) is already faithful
-// (|a*y - a/b| <= |a/b| 2^-54 < ulp(a/b)/2, so q0 is one of the two doubles
-// around a/b, also where a*y and a/b lie in different binades), and one
-// Markstein step q1 = q0 - (b*q0 - a)*y gives RN(a/b).  The generator checks
-// the condition in exact rational arithmetic (b = RN(1e-4) with y = 1e4:
-// |b*y - 1| = 0.043 * 2^-53).  Range handling as in fmr_divc.
-__host__ __device__ __forceinline__ double fmr_divc1(double a, double b, double y) {
-  const double aa = fabs(a);
-  if (__builtin_expect(!((aa >= 0x1p-900 && aa <= 0x1p+900) || aa == 0.0), 0)) return a / b;
+// One correction, for divisors whose host-rounded reciprocal is close enough:
+// if |b*y - 1| <= 2^-54 then q0 = RN(a*y) is already faithful (|a*y - a/b| <=
+// |a/b| 2^-54 < ulp(a/b)/2, so q0 is one of the two doubles around a/b, also
+// where a*y and a/b lie in different binades), and one Markstein step
+// q1 = q0 - (b*q0 - a)*y gives RN(a/b).  The generator checks the condition in
+// exact rational arithmetic (b = RN(1e-4) with y = 1e4: |b*y - 1| = 0.043 * 2^-53).
+__host__ __device__ __forceinline__ double fmr_divc1_fast(double a, double b, double y) {
   const double q = a * y;
   const double r = fma(b, q, -a);
   return fma(-r, y, q);
+}
+
+__host__ __device__ __forceinline__ double fmr_divc(double a, double b, double y) {
+  if (__builtin_expect(!fmr_divc_in_range(a), 0)) return a / b;
+  return fmr_divc_fast(a, b, y);
+}
+
+__host__ __device__ __forceinline__ double fmr_divc1(double a, double b, double y) {
+  if (__builtin_expect(!fmr_divc_in_range(a), 0)) return a / b;
+  return fmr_divc1_fast(a, b, y);
+}
+
+// Speculative forms for point functions: the fused sequences without the
+// branch; the lanes whose dividend is not fmr_divc_in_range are or-ed into the
+// wave mask `bad` (one ballot per comparison, each a single compare into a lane
+// mask: !in_range = (!(|a| >= 2^-900) || !(|a| <= 2^900)) && a != 0).  A caller
+// evaluates the whole point with these and, if its wave's mask came out
+// nonzero, evaluates it again with fmr_divc / fmr_divc1 (a branch on a
+// wave-uniform value).  Without a branch per division the compiler can issue
+// all of a point's loads before its arithmetic.
+__device__ __forceinline__ unsigned long long fmr_divc_out_of_range(double a) {
+  const double aa = fabs(a);
+  return (__builtin_amdgcn_ballot_w64(!(aa >= 0x1p-900)) | __builtin_amdgcn_ballot_w64(!(aa <= 0x1p+900))) &
+         __builtin_amdgcn_ballot_w64(aa != 0.0);
+}
+
+__device__ __forceinline__ double fmr_divcs(double a, double b, double y, unsigned long long &bad) {
+  bad |= fmr_divc_out_of_range(a);
+  return fmr_divc_fast(a, b, y);
+}
+
+__device__ __forceinline__ double fmr_divc1s(double a, double b, double y, unsigned long long &bad) {
+  bad |= fmr_divc_out_of_range(a);
+  return fmr_divc1_fast(a, b, y);
 }
 
 // Positivity checks of the mixed-precision form: every FP64 square-root

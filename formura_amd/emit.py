@@ -185,6 +185,15 @@ def _divc_ok(b) -> bool:
     return math.isfinite(b) and 2.0 ** -100 <= abs(b) <= 2.0 ** 100
 
 
+def _divfix_lines(ct, name, src, sg, fn, b, y, cast):
+    """A division by a constant as the branch-free fused sequence, computed
+    again with the guarded form where a lane of the wave holds a dividend
+    outside its range (a wave-uniform branch)."""
+    return ["%s %s = %s(%s%s_fast(%s, %s, %s));" % (ct, name, cast, sg, fn, src, b, y),
+            "if (__builtin_expect(fmr_divc_out_of_range(%s) != 0ull, 0)) %s = %s(%s%s(%s, %s, %s));"
+            % (src, name, cast, sg, fn, src, b, y)]
+
+
 def _const_literal(v, typ):
     if typ == "bool":
         return "true" if v else "false"
@@ -199,7 +208,7 @@ class Body:
     def __init__(self, node: MMNode, load: Callable, tmp_load: Callable, index: Callable,
                  store: Callable, prefix: str, fold: bool = True, const_kw="const ",
                  params: Dict[str, str] = None, real: str = None, recip: bool = False,
-                 prec: Dict[int, str] = None, divc: bool = True):
+                 prec: Dict[int, str] = None, divc: bool = True, spec_ok: str = None, divfix: bool = False):
         """``params``: literal names bound to run-time expressions (not folded);
         ``real``: if set ("float"), every floating micro value is computed in
         that type (the FP32 form; constants are folded in double and rounded,
@@ -215,8 +224,19 @@ class Body:
         reciprocal and two fused corrections) or, for a power-of-two
         divisor, as the exact product with its reciprocal; host code keeps
         the division (with one correction instead of two where the divisor
-        allows it: ``fmr_divc1``)."""
+        allows it: ``fmr_divc1``);
+        ``spec_ok``: name of a wave mask the caller declared zero: such
+        divisions use the branch-free forms (``fmr_divcs``/``fmr_divc1s``),
+        which set the bits of lanes whose dividend lies outside their range --
+        the caller then evaluates the node again without ``spec_ok``;
+        ``divfix``: such divisions use the branch-free sequence and, where a
+        lane of the wave holds a dividend outside its range, are computed
+        again with the guarded form (a branch on a wave-uniform value, the
+        division alone redone)."""
         self.lines: List[str] = []
+        sfx = (lambda fn: fn + "s") if spec_ok else (lambda fn: fn)
+        sarg = (", " + spec_ok) if spec_ok else ""
+        self.spec_used = False
         vals: Dict[int, object] = {}
         params = params or {}
         ctypes: Dict[int, str] = {}
@@ -263,9 +283,14 @@ class Body:
                 if mant == 0.5:
                     rhs = "%s*%s" % (src, show_double(1.0 / b))
                 else:
-                    fn = "fmr_divc1" if _divc_one_step(abs(b)) else "fmr_divc"
+                    fn = sfx("fmr_divc1" if _divc_one_step(abs(b)) else "fmr_divc")
+                    self.spec_used |= bool(spec_ok)
                     sg = "" if b > 0 else "-"
-                    rhs = "%s%s(%s, %s, %s)" % (sg, fn, src, show_double(abs(b)), show_double(1.0 / abs(b)))
+                    rhs = "%s%s(%s, %s, %s%s)" % (sg, fn, src, show_double(abs(b)), show_double(1.0 / abs(b)), sarg)
+                    if divfix:
+                        self.lines += _divfix_lines(ct, name, src, sg, fn, show_double(abs(b)),
+                                                    show_double(1.0 / abs(b)), "(float)")
+                        continue
                 self.lines.append("%s%s %s = (float)(%s);" % (const_kw, ct, name, rhs))
                 continue
             if ((real or recip or (ct == "float" and prec)) and k == "Binop" and m.inst[1] == "/" and m.inst[3] in vals
@@ -288,11 +313,16 @@ class Body:
                     # 1/b is exact: a*(1/b) and a/b are the same real number, rounded once
                     rhs = "%s*%s" % (src, show_double(1.0 / b))
                 else:
-                    fn = "fmr_divc1" if _divc_one_step(abs(b)) else "fmr_divc"
+                    fn = sfx("fmr_divc1" if _divc_one_step(abs(b)) else "fmr_divc")
+                    self.spec_used |= bool(spec_ok)
                     if b > 0:
-                        rhs = "%s(%s, %s, %s)" % (fn, src, show_double(b), show_double(1.0 / b))
+                        rhs = "%s(%s, %s, %s%s)" % (fn, src, show_double(b), show_double(1.0 / b), sarg)
                     else:
-                        rhs = "-%s(%s, %s, %s)" % (fn, src, show_double(-b), show_double(-1.0 / b))
+                        rhs = "-%s(%s, %s, %s%s)" % (fn, src, show_double(-b), show_double(-1.0 / b), sarg)
+                    if divfix:
+                        self.lines += _divfix_lines(ct, name, src, "" if b > 0 else "-", fn, show_double(abs(b)),
+                                                    show_double(1.0 / abs(b)), "")
+                        continue
                 self.lines.append("%s%s %s = %s;" % (const_kw, ct, name, rhs))
                 continue
             if fold:
@@ -607,8 +637,9 @@ def tile_load(count: str, index_lines: List[str], pairs: List[tuple], ctype: str
     ``idx`` and ``g``; all are of C type ``ctype``.  ``max_count`` (a constant
     expression) bounds ``count`` when ``count`` is not itself constant."""
     it = "%s_IT" % tag
-    out = ["%sconstexpr int %s = (%s + %d) / %d;" % (indent, it, max_count or count, threads - 1, threads)]
-    out.append("%s%s %s;" % (indent, ctype, ", ".join("%s_v%d[%s]" % (tag, i, it) for i in range(len(pairs)))))
+    decl = ["%sconstexpr int %s = (%s + %d) / %d;" % (indent, it, max_count or count, threads - 1, threads)]
+    decl.append("%s%s %s;" % (indent, ctype, ", ".join("%s_v%d[%s]" % (tag, i, it) for i in range(len(pairs)))))
+    out = []
     out.append("%s#pragma unroll" % indent)
     out.append("%sfor (int it = 0; it < %s; ++it) {" % (indent, it))
     out.append("%s  const int idx = threadIdx.x + it * %d;" % (indent, threads))
@@ -617,6 +648,7 @@ def tile_load(count: str, index_lines: List[str], pairs: List[tuple], ctype: str
     out += ["%s    %s_v%d[it] = %s;" % (indent, tag, i, src) for i, (dst, src) in enumerate(pairs)]
     out.append("%s  }" % indent)
     out.append("%s}" % indent)
+    issue, out = out, []
     out.append("%s#pragma unroll" % indent)
     out.append("%sfor (int it = 0; it < %s; ++it) {" % (indent, it))
     out.append("%s  const int idx = threadIdx.x + it * %d;" % (indent, threads))
@@ -624,4 +656,4 @@ def tile_load(count: str, index_lines: List[str], pairs: List[tuple], ctype: str
     out += ["%s    %s = %s_v%d[it];" % (indent, dst, tag, i) for i, (dst, src) in enumerate(pairs)]
     out.append("%s  }" % indent)
     out.append("%s}" % indent)
-    return "\n".join(out)
+    return "\n".join(decl + issue + out)

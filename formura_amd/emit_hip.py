@@ -21,6 +21,7 @@ Kernel schedules (chosen per program, see ``choose_schedule``):
 from __future__ import annotations
 
 import json
+import re
 from typing import Dict, List
 
 from . import syntax as S
@@ -500,6 +501,13 @@ extern "C" long long Formura_HIP_Positivity_Violations(void) { return fmr_rt::vi
 TEMP_GROUP_MICRO = 20000  # instructions of one merged temporary kernel (before sharing): one kernel per level in practice
 
 
+_LDS_LOAD = re.compile(r"^const \w+ \w+ = lds\d+\[[^\]]*\];$")
+
+
+def _indent(text: str, pre: str) -> str:
+    return "\n".join(pre + l if l.strip() else l for l in text.split("\n"))
+
+
 def temp_groups(plan):
     """Manifest temporaries in launch order, grouped: by dependency level (a
     temporary's level is one more than the highest level it reads; state-only
@@ -582,8 +590,28 @@ class Tile2DSchedule:
 
     name = "tile2d"
 
-    def __init__(self, tx, ty, k, inplace=False, threads=256, pair=False):
+    def __init__(self, tx, ty, k, inplace=False, threads=256, pair=False, div="fix", colmap=False, vpair=False):
         self.tx, self.ty, self.k = tx, ty, k
+        # div: how divisions by constants are guarded against dividends outside
+        # the range of the fused sequence.  "spec": no branch per division
+        # (fmr_divc1s); a wave with such a dividend evaluates the point again
+        # with the guarded forms (one wave-uniform branch per point).  "fix":
+        # each division redone with the guarded form where its wave needs it
+        # (a wave-uniform branch per division).  "guard": a per-lane branch in
+        # each division (fmr_divc1).  In every form the point's loads are
+        # issued first.
+        if div not in ("spec", "fix", "guard"):
+            raise EmitError("div must be spec, fix or guard")
+        self.div = div
+        # colmap: a level's work-items form P = NTH / W row groups of W columns
+        # (W = the level's width); each owns one column and steps down rows
+        # P apart, so a point's LDS offset advances by a constant instead of
+        # being decomposed from a flat index
+        self.colmap = bool(colmap)
+        # vpair: each work-item evaluates two vertically adjacent points (rows
+        # r, r+1 of one column): the column values they share are loaded once
+        self.vpair = bool(vpair)
+
         # work-items per workgroup (a level's points are dealt round-robin)
         self.threads = int(threads)
         # pair: each work-item evaluates two adjacent points of a row; a level's
@@ -607,28 +635,43 @@ class Tile2DSchedule:
         params = ", ".join("const %s* __restrict__ lds%d" % (ct, i) for i, (n, ct) in enumerate(fields))
         outs = ", ".join("%s& o%d" % (ct, i) for i, (n, ct) in enumerate(fields))
         tbi = getattr(em, "tb_index", False)
-        for k in plan.stores:
-            node = plan.mm[k]
-            r = plan.rng[k]
-            base = r.offset - 2 * s
 
-            def load(field, c, base=base):
-                fi = em.field_index[field]
-                dr, dc = c[0] + base, c[1] + base
-                return "lds%d[o + (%d)*RY + (%d)]" % (fi, dr, dc)
+        def bodies(spec_ok):
+            loads, lines, used = [], [], False
+            for k in plan.stores:
+                node = plan.mm[k]
+                r = plan.rng[k]
+                base = r.offset - 2 * s
 
-            def store(fname, v):
-                return "o%d = %s;" % (em.field_index[fname], v)
+                def load(field, c, base=base):
+                    fi = em.field_index[field]
+                    dr, dc = c[0] + base, c[1] + base
+                    return "lds%d[o + (%d)*RY + (%d)]" % (fi, dr, dc)
 
-            def index(ax):
-                # unwrapped tile coordinate + 2s(nt - j) + the Forward's offset
-                return "(%s + ib%d)" % ("gx" if ax == 0 else "gy", ax)
+                def store(fname, v):
+                    return "o%d = %s;" % (em.field_index[fname], v)
 
-            b = Body(node, load, lambda *a: None, index if tbi else (lambda ax: None), store, "s%d_" % k,
-                     prec=em.prec_of(node), recip=em.contract)
-            body_lines.append("{")
-            body_lines += ["  " + l for l in b.lines]
-            body_lines.append("}")
+                def index(ax):
+                    # unwrapped tile coordinate + 2s(nt - j) + the Forward's offset
+                    return "(%s + ib%d)" % ("gx" if ax == 0 else "gy", ax)
+
+                b = Body(node, load, lambda *a: None, index if tbi else (lambda ax: None), store, "s%d_" % k,
+                         prec=em.prec_of(node), recip=em.contract, spec_ok=spec_ok,
+                         divfix=self.div == "fix" and spec_ok is None)
+                used |= b.spec_used
+                # the node's loads (names unique to the node) before every node's arithmetic
+                loads += [l for l in b.lines if _LDS_LOAD.search(l)]
+                lines.append("{")
+                lines += ["  " + l for l in b.lines if not _LDS_LOAD.search(l)]
+                lines.append("}")
+            return loads + lines, used
+
+        body_lines, used = bodies("bad_" if self.div == "spec" else None)
+        if self.div == "spec" and used:
+            exact, _ = bodies(None)
+            body_lines = (["unsigned long long bad_ = 0;  // lanes with a dividend out of range"] + body_lines
+                          + ["if (__builtin_expect(bad_ != 0, 0)) {  // the wave again, with guarded divisions"]
+                          + ["  " + l for l in exact] + ["}"])
         ipar = ", const int gx, const int gy, const int ib0, const int ib1" if tbi else ""
         out.append("template <int RY>\n__device__ __forceinline__ void pt_step(%s, const int o%s, %s) {\n%s\n}\n"
                    % (params, ipar, outs, "\n".join("  " + l for l in body_lines)))
@@ -661,55 +704,35 @@ class Tile2DSchedule:
                                           decl_o=decl_o + " " + decl_p, srcs=srcs0, refs_o=refs_o, refs_p=refs_p,
                                           st_lds=st_lds2, st_glb=st_glb2))
             return out
+        ibdecl = ("\n    // index reads: the value at unwrapped tile coordinate g of level j of the\n"
+                  "    // Forward reads g + 2s(K - j) + the offset at the Forward's start\n"
+                  "    const int ib0 = 2 * SLEEVE * (K - j) + offT0, ib1 = 2 * SLEEVE * (K - j) + offT1;"
+                  if tbi else "")
         if self.inplace:
-            out.append(TILE2D_INPLACE % dict(
-                TX=TX, TY=TY, NTH=self.threads, lds=lds, load_tile=load_tile, decl_o=decl_o, refs_o=refs_o,
+            if self.colmap:
+                out.append(TILE_ROWS_FN)
+            levels = (LEVELS_INPLACE_COLS if self.colmap else LEVELS_INPLACE) % dict(
+                NTH=self.threads, decl_o=decl_o, refs_o=refs_o,
                 srcs=", ".join("B%d[0]" % i for i in range(len(fields))),
                 rdecl=" ".join("%s res%d[NI];" % (ct, i) for i, (n, ct) in enumerate(fields)),
                 rset=" ".join("res%d[it] = o%d;" % (i, i) for i in range(len(fields))),
                 rput=" ".join("B%d[0][o] = res%d[it];" % (i, i) for i in range(len(fields))),
-                st_glb=st_glb))
-            return out
-        out.append("""
-template <int K>
-__global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out%(offp)s) {
-  constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
-  constexpr int NTX = N1 / TX, NTY = N2 / TY, NT = NTX * NTY;
-%(lds)s
-  // XCD-aware tile order: workgroup ids are dealt round-robin to the 8 XCDs, so
-  // give each XCD a contiguous band of tiles (shared halos stay in its L2).
-  const int b = blockIdx.x;
-  const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
-  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
-%(load_tile)s
-  __syncthreads();
-#pragma unroll
-  for (int j = 1; j <= K; ++j) {
-    const int src = (j - 1) & 1, dst = j & 1;
-    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;%(ibdecl)s
-    for (int idx = threadIdx.x; idx < HH * W; idx += %(NTH)d) {
-      const int rr = idx / W;
-      const int r = lo + rr, c = lo + (idx - rr * W);
-      const int o = r * RY + c;
-      %(decl_o)s
-      pt_step<RY>(%(srcs)s, %(refs_o)s);
-      if (j < K) {
-        %(st_lds)s
-      } else {
-        const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
-        %(st_glb)s
-      }
-    }
-    if (j < K) __syncthreads();
-  }
-}
-""" % dict(TX=TX, TY=TY, NTH=self.threads, lds=lds, wrap1=wrap1, wrap2=wrap2, load_tile=load_tile, decl_o=decl_o,
-           srcs=srcs, refs_o=refs_o, st_lds=st_lds, st_glb=st_glb,
-           offp=", const int offT0, const int offT1" if tbi else "",
-           ibdecl=("\n    // index reads: the value at unwrapped tile coordinate g of level j of the\n"
-                   "    // Forward reads g + 2s(K - j) + the offset at the Forward's start\n"
-                   "    const int ib0 = 2 * SLEEVE * (K - j) + offT0, ib1 = 2 * SLEEVE * (K - j) + offT1;")
-           if tbi else ""))
+                st_glb=st_glb)
+        elif self.vpair:
+            decl_p = " ".join("%s p%d;" % (ct, i) for i, (n, ct) in enumerate(fields))
+            refs_p = ", ".join("p%d" % i for i in range(len(fields)))
+            srcs0 = ", ".join("B%d[src]" % i for i in range(len(fields)))
+            levels = LEVELS_PINGPONG % dict(ibdecl=ibdecl, loop=TILE2D_VPAIR_LOOP % dict(
+                NTH=self.threads, decl_o=decl_o + " " + decl_p, srcs=srcs0, refs_o=refs_o, refs_p=refs_p,
+                st_lds=st_lds + " " + " ".join("B%d[dst][o + RY] = p%d;" % (i, i) for i in range(len(fields))),
+                st_glb=st_glb + " " + " ".join("FLD%d(st_out)[g + N2] = p%d;" % (i, i) for i in range(len(fields)))))
+        else:
+            levels = LEVELS_PINGPONG % dict(ibdecl=ibdecl, loop=(TILE2D_COLS_LOOP if self.colmap else TILE2D_FLAT_LOOP)
+                                            % dict(NTH=self.threads, decl_o=decl_o, srcs=srcs, refs_o=refs_o,
+                                                   st_lds=st_lds, st_glb=st_glb))
+        head = TILE2D_HEAD % dict(NTH=self.threads, TX=TX, TY=TY, lds=lds,
+                                  offp=", const int offT0, const int offT1" if tbi else "")
+        out.append(head + TILE2D_ONE % dict(load_tile=load_tile, levels=levels))
         return out
 
     def launch(self, em, plan):
@@ -740,6 +763,183 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ 
                 "    const int k = steps >= %d ? %d : (int)steps;\n"
                 "    run_step_tile(k, g_rt.cur, g_rt.nxt, g_rt.stream); g_rt.swap();\n"
                 "    g_rt.launches += 1; steps -= k;\n  }\n}" % (self.k, self.k))
+
+
+TILE2D_FLAT_LOOP = """    for (int idx = threadIdx.x; idx < HH * W; idx += %(NTH)d) {
+      const int rr = idx / W;
+      const int r = lo + rr, c = lo + (idx - rr * W);
+      const int o = r * RY + c;
+      %(decl_o)s
+      pt_step<RY>(%(srcs)s, %(refs_o)s);
+      if (j < K) {
+        %(st_lds)s
+      } else {
+        const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+        %(st_glb)s
+      }
+    }"""
+
+# work-item pairs of rows: pair (rp, c) is the points (lo + 2 rp, c) and
+# (lo + 2 rp + 1, c); a level's height RX - lo is even (TX is)
+TILE2D_VPAIR_LOOP = """    static_assert((RX - 2 * SLEEVE) %% 2 == 0, "a level's rows must pair up");
+    for (int idx = threadIdx.x; idx < (HH / 2) * W; idx += %(NTH)d) {
+      const int rp = idx / W;
+      const int r = lo + 2 * rp, c = lo + (idx - rp * W);
+      const int o = r * RY + c;
+      %(decl_o)s
+      pt_step<RY>(%(srcs)s, o, %(refs_o)s);
+      pt_step<RY>(%(srcs)s, o + RY, %(refs_p)s);
+      if (j < K) {
+        %(st_lds)s
+      } else {
+        const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+        %(st_glb)s
+      }
+    }"""
+
+# work-item t < P*W owns column lo + t %% W of rows lo + t / W + n*P (P = NTH / W
+# row groups; the level's width W is at most NTH)
+TILE2D_COLS_LOOP = """    static_assert(RY - 2 * SLEEVE <= %(NTH)d, "a level's columns must fit the workgroup");
+    const int P = %(NTH)d / W;
+    (void)HH;
+    if ((int)threadIdx.x < P * W) {
+      const int rg = (int)threadIdx.x / W;
+      const int c = lo + ((int)threadIdx.x - rg * W);
+      int o = (lo + rg) * RY + c;
+      for (int r = lo + rg; r < RX; r += P, o += P * RY) {
+        %(decl_o)s
+        pt_step<RY>(%(srcs)s, %(refs_o)s);
+        if (j < K) {
+          %(st_lds)s
+        } else {
+          const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+          %(st_glb)s
+        }
+      }
+    }"""
+
+
+TILE_ROWS_FN = """
+// most rows a work-item evaluates in one level of the column mapping
+__host__ __device__ constexpr int fmr_tile_rows(int RX, int RY, int S, int K, int NTH) {
+  int m = 0;
+  for (int j = 1; j <= K; ++j) {
+    const int lo = 2 * j * S, P = NTH / (RY - lo), n = (RX - lo + P - 1) / P;
+    m = n > m ? n : m;
+  }
+  return m;
+}
+"""
+
+TILE2D_HEAD = """
+template <int K>
+__global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out%(offp)s) {
+  constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
+  constexpr int NTX = N1 / TX, NTY = N2 / TY, NT = NTX * NTY;
+%(lds)s
+"""
+
+# one tile per workgroup
+TILE2D_ONE = """  // XCD-aware tile order: workgroup ids are dealt round-robin to the 8 XCDs, so
+  // give each XCD a contiguous band of tiles (shared halos stay in its L2).
+  const int b = blockIdx.x;
+  const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
+  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+%(load_tile)s
+  __syncthreads();
+%(levels)s
+}
+"""
+
+# K levels, ping-pong LDS buffers (level j reads buffer (j-1)&1, writes j&1)
+LEVELS_PINGPONG = """#pragma unroll
+  for (int j = 1; j <= K; ++j) {
+    const int src = (j - 1) & 1, dst = j & 1;
+    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;%(ibdecl)s
+%(loop)s
+    if (j < K) __syncthreads();
+  }"""
+
+# K levels in one LDS buffer: a level's values go to registers and overwrite
+# the previous level after a barrier
+LEVELS_INPLACE = """  // points of the largest level per thread; level j covers [2js, RX) x [2js, RY)
+  constexpr int NI = ((RX - 2 * SLEEVE) * (RY - 2 * SLEEVE) + %(NTH)d - 1) / %(NTH)d;
+#pragma unroll
+  for (int j = 1; j <= K; ++j) {
+    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
+    %(rdecl)s
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      const int idx = threadIdx.x + it * %(NTH)d;
+      if (idx < HH * W) {
+        const int rr = idx / W;
+        const int r = lo + rr, c = lo + (idx - rr * W);
+        const int o = r * RY + c;
+        %(decl_o)s
+        pt_step<RY>(%(srcs)s, o, %(refs_o)s);
+        if (j < K) {
+          %(rset)s
+        } else {
+          const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+          %(st_glb)s
+        }
+      }
+    }
+    if (j < K) {
+      __syncthreads();  // every read of level j-1 is done
+#pragma unroll
+      for (int it = 0; it < NI; ++it) {
+        const int idx = threadIdx.x + it * %(NTH)d;
+        if (idx < HH * W) {
+          const int rr = idx / W;
+          const int o = (lo + rr) * RY + lo + (idx - rr * W);
+          %(rput)s
+        }
+      }
+      __syncthreads();
+    }
+  }"""
+
+LEVELS_INPLACE_COLS = """  static_assert(RY - 2 * SLEEVE <= %(NTH)d, "a level's columns must fit the workgroup");
+  constexpr int NI = fmr_tile_rows(RX, RY, SLEEVE, K, %(NTH)d);
+#pragma unroll
+  for (int j = 1; j <= K; ++j) {
+    // level j covers [lo, RX) x [lo, RY): P = NTH / W row groups of W columns;
+    // work-item t < P*W owns column lo + t %% W of rows lo + t / W + it*P
+    const int lo = 2 * j * SLEEVE, W = RY - lo, P = %(NTH)d / W, NJ = (RX - lo + P - 1) / P;
+    const int t = threadIdx.x;
+    const bool on = t < P * W;
+    const int rg = on ? t / W : 0;
+    const int c = lo + (t - rg * W);
+    %(rdecl)s
+#pragma unroll
+    for (int it = 0; it < NJ; ++it) {
+      const int r = lo + rg + it * P;
+      if (on && r < RX) {
+        const int o = r * RY + c;
+        %(decl_o)s
+        pt_step<RY>(%(srcs)s, o, %(refs_o)s);
+        if (j < K) {
+          %(rset)s
+        } else {
+          const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
+          %(st_glb)s
+        }
+      }
+    }
+    if (j < K) {
+      __syncthreads();  // every read of level j-1 is done
+#pragma unroll
+      for (int it = 0; it < NJ; ++it) {
+        const int r = lo + rg + it * P;
+        if (on && r < RX) {
+          const int o = r * RY + c;
+          %(rput)s
+        }
+      }
+      __syncthreads();
+    }
+  }"""
 
 
 TILE2D_PAIR = """
@@ -782,60 +982,6 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ 
     }
     (void)HH;
     if (j < K) __syncthreads();
-  }
-}
-"""
-
-
-TILE2D_INPLACE = """
-template <int K>
-__global__ void __launch_bounds__(%(NTH)d) k_step_tile(const char* __restrict__ st_in, char* __restrict__ st_out) {
-  constexpr int TX = %(TX)d, TY = %(TY)d, H = 2 * K * SLEEVE, RX = TX + H, RY = TY + H;
-  constexpr int NTX = N1 / TX, NTY = N2 / TY, NT = NTX * NTY;
-%(lds)s
-  // XCD-aware tile order: workgroup ids are dealt round-robin to the 8 XCDs, so
-  // give each XCD a contiguous band of tiles (shared halos stay in its L2).
-  const int b = blockIdx.x;
-  const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
-  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
-%(load_tile)s
-  __syncthreads();
-  // points of the largest level per thread; level j covers [2js, RX) x [2js, RY)
-  constexpr int NI = ((RX - 2 * SLEEVE) * (RY - 2 * SLEEVE) + %(NTH)d - 1) / %(NTH)d;
-#pragma unroll
-  for (int j = 1; j <= K; ++j) {
-    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;
-    %(rdecl)s
-#pragma unroll
-    for (int it = 0; it < NI; ++it) {
-      const int idx = threadIdx.x + it * %(NTH)d;
-      if (idx < HH * W) {
-        const int rr = idx / W;
-        const int r = lo + rr, c = lo + (idx - rr * W);
-        const int o = r * RY + c;
-        %(decl_o)s
-        pt_step<RY>(%(srcs)s, o, %(refs_o)s);
-        if (j < K) {
-          %(rset)s
-        } else {
-          const long long g = (long long)(x0 + r - H) * N2 + (y0 + c - H);
-          %(st_glb)s
-        }
-      }
-    }
-    if (j < K) {
-      __syncthreads();  // every read of level j-1 is done
-#pragma unroll
-      for (int it = 0; it < NI; ++it) {
-        const int idx = threadIdx.x + it * %(NTH)d;
-        if (idx < HH * W) {
-          const int rr = idx / W;
-          const int o = (lo + rr) * RY + lo + (idx - rr * W);
-          %(rput)s
-        }
-      }
-      __syncthreads();
-    }
   }
 }
 """
@@ -1551,8 +1697,15 @@ def schedule_from_spec(em, spec: dict):
                                  > int(spec.get("threads", 256))):
             raise EmitError("tile2d pairs need an even tile width, a row's pairs within the workgroup, "
                             "ping-pong buffers and no index reads")
+        if spec.get("vpair") and (int(spec["tx"]) % 2 or spec.get("inplace") or spec.get("colmap") or spec.get("pair")
+                                  or getattr(em, "tb_index", False)):
+            raise EmitError("row pairs need an even tile height, ping-pong buffers and no index reads")
+        if spec.get("colmap") and (int(spec["ty"]) + 2 * (int(spec["k"]) - 1) * em.step.sleeve
+                                   > int(spec.get("threads", 256))):
+            raise EmitError("the column mapping needs a level's width within the workgroup")
         return Tile2DSchedule(*args, inplace=bool(spec.get("inplace", 0)), threads=int(spec.get("threads", 256)),
-                              pair=bool(spec.get("pair", 0)))
+                              pair=bool(spec.get("pair", 0)), div=spec.get("div", "fix"),
+                              colmap=bool(spec.get("colmap", 0)), vpair=bool(spec.get("vpair", 0)))
     if kind == "wave2d":
         args = (int(spec["rows"]), int(spec["k"]))
         if not _wave2d_ok(em, *args):
@@ -1623,6 +1776,10 @@ def candidate_specs(em) -> List[dict]:
                             if nth != 256:
                                 spec["threads"] = nth
                             out.append(spec)
+                            if not inplace and ty + 2 * (k - 1) * em.step.sleeve <= nth:
+                                out.append(dict(spec, colmap=1))
+                            if not inplace and tx % 2 == 0 and not getattr(em, "tb_index", False):
+                                out.append(dict(spec, vpair=1))
                             if not inplace and ty % 2 == 0 and not getattr(em, "tb_index", False):
                                 out.append(dict(spec, pair=1))
         for rows in (8, 16, 32, 64):

@@ -6,6 +6,7 @@
 // next to a rounding midpoint (the hard cases of correct rounding), stencil-like
 // differences, and specials (signed zeros, subnormals, the range limits, Inf, NaN).
 #include "formura_hip_rt.h"
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -47,6 +48,12 @@ int main(int argc, char** argv) {
       double q1 = fmr_divc(a, b, y), q2 = a / b;
       ++tot;
       if (bits(q1) != bits(q2)) { if (bad < 10) printf("MISMATCH b=%a a=%a got %a want %a\n", b, a, q1, q2); ++bad; }
+      if (fmr_divc_in_range(a)) {  // the branch-free sequence the speculative forms use
+        const double q4 = fmr_divc_fast(a, b, y);
+        ++tot;
+        if (bits(q4) != bits(q2)) { if (bad < 10) printf("MISMATCH (fast) b=%a a=%a got %a want %a\n", b, a, q4, q2); ++bad; }
+        if (one && bits(fmr_divc1_fast(a, b, y)) != bits(q2)) { if (bad < 10) printf("MISMATCH (fast, one step) b=%a a=%a\n", b, a); ++bad; }
+      }
       if (one) {
         const double q3 = fmr_divc1(a, b, y);
         ++tot;
@@ -54,6 +61,11 @@ int main(int argc, char** argv) {
       }
     }
     double sp[] = {0.0, -0.0, 1e-310, -1e-310, 1e308, -1e308, INFINITY, -INFINITY, NAN, 0x1p-900, 0x1p900, 0x1p-901, 0x1.fffffffffffffp900, 5e-324};
+    // the range test on the specials: in range exactly the signed zeros and 2^-900, 2^900
+    for (double a : sp) {
+      const bool want = a == 0.0 || fabs(a) == 0x1p-900 || fabs(a) == 0x1p900;
+      if (fmr_divc_in_range(a) != want) { printf("RANGE MISMATCH a=%a\n", a); ++bad; }
+    }
     for (double a : sp) {
       double q1 = fmr_divc(a, b, y), q2 = a / b, q3 = one ? fmr_divc1(a, b, y) : q2;
       ++tot;

@@ -31,8 +31,7 @@ def checker(tmp_path_factory):
     cmd = [build.HIPCC, "-O2", "-mfma", "-ffp-contract=off", "-I", build.CSRC,
            os.path.join(HERE, "native", "divc_check.cpp"), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        pytest.skip("host build of the division check failed: %s" % r.stderr[-500:])
+    assert r.returncode == 0, "host build of the division check failed: %s" % r.stderr[-1500:]
     return out
 
 
@@ -45,11 +44,19 @@ def test_divc_matches_ieee_division(checker, seed):
 
 def test_emitted_divisions():
     """A non-power-of-two constant divisor becomes fmr_divc with its rounded
-    reciprocal; a power-of-two one an exact product; host Setup keeps '/'."""
+    reciprocal; a power-of-two one an exact product; host Setup keeps '/'.
+    The tile kernels' point function divides with the branch-free forms and
+    holds the guarded ones for the waves that need them."""
     from formura_amd.emit_hip import generate
     _, src, _ = generate(os.path.join(build.PROG_DIR, "grayscott2d.fmr"),
                          os.path.join(build.PROG_DIR, "gs_64.yaml"), "gs_64")
-    assert src.count("fmr_divc1(") == 4 and src.count(", 0.0001, 10000.0)") == 4  # d_xx and d_yy of u and of v
+    # d_xx and d_yy of u and of v: the fused sequence, and the guarded form
+    # for the waves that hold a dividend outside its range
+    assert src.count("fmr_divc1_fast(") == 4 and src.count("fmr_divc1(") == 4
+    assert src.count(", 0.0001, 10000.0)") == 8 and src.count("fmr_divc_out_of_range(") == 4
+    _, fsrc, _ = generate(os.path.join(build.PROG_DIR, "grayscott2d.fmr"),
+                          os.path.join(build.PROG_DIR, "gs_64.yaml"), "gs_64", "flat")
+    assert fsrc.count("fmr_divc1(") == 4 and "fmr_divc1s(" not in fsrc
     setup = src[src.index("static void Formura_Setup"):]
     assert "fmr_divc" not in setup[:setup.index("\n}\n")]
     _, hsrc, _ = generate(os.path.join(build.PROG_DIR, "heat3d.fmr"),

@@ -84,6 +84,48 @@ def test_grayscott_bit_exact(name, steps):
     _assert_same(o, h)
 
 
+def _gs_extreme_seed(lib):
+    """Gray-Scott state whose Laplacian numerators leave the range where the
+    branch-free constant division is correctly rounded (|a| < 2^-900: bands of
+    v = 1e-300 and of subnormal v = 1e-310 next to each other, which stay tiny
+    as v decays; |a| > 2^900: a band of u = 1e275; exact zeros inside the
+    bands), among ordinary values: the waves that hold such a point are
+    evaluated again with guarded divisions.  v is exactly 0 around the huge
+    band until v's front reaches it (after 11 steps of this 64^2 run), so the
+    reaction term u*v*v stays finite for the steps run."""
+    st = _gs_seed(lib)
+    u, v = st["u"], st["v"]
+    assert u.shape[0] == 64 and not v[:26].any() and not v[39:].any()
+    v[8:11, :] = 1e-300
+    v[11:13, :] = 1e-310
+    u[55:58, ::3] = 1e275
+    return {"u": u, "v": v}
+
+
+@pytest.mark.parametrize("name,oracle,steps,resident", [
+    ("gs_64", "gs_64", 9, False),               # one step per launch (tile2d, 1 level)
+    ("gs_64", "gs_64", 10, True),               # 4 steps per launch
+    ("gs_64_tile_k5_inplace", "gs_64", 10, True),
+    ("gs_64_tile_spec", "gs_64", 10, True),     # such waves evaluate the whole point again
+    ("gs_64_tile_guard", "gs_64", 10, True),    # a per-lane branch in each division
+    ("gs_64_tile_colmap", "gs_64", 10, True),
+    ("gs_64_tile_vpair", "gs_64", 10, True),
+    ("gs_64_flat", "gs_64", 7, False),          # the guarded divisions of the flat kernels
+])
+def test_grayscott_out_of_range_dividends(name, oracle, steps, resident):
+    """Bit for bit (signed zeros included) with the oracle's IEEE division where
+    the dividends are tiny, subnormal, huge or zero."""
+    print("%s: %d steps, resident %s" % (name, steps, resident), flush=True)
+    o, h = _run_pair(name, oracle, _gs_extreme_seed, steps, resident=resident)
+    assert o.navi_dict() == h.navi_dict()
+    u, v = o.field("u"), o.field("v")
+    assert np.all(np.isfinite(u)) and np.all(np.isfinite(v)) and (np.abs(u) > 1e270).any()
+    assert ((v != 0) & (np.abs(v) < 1e-300)).any() and ((v != 0) & (np.abs(v) < 1e-307)).any()
+    for f in o.fields:
+        a, b = o.field(f), h.field(f)
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f
+
+
 @pytest.mark.parametrize("name,steps", [("gs_64", 50), ("gs_2048", 43)])
 def test_resident_multi_step_launches(name, steps):
     """Formura_Forward_Steps runs several steps per launch (temporal blocking in LDS)."""
@@ -105,6 +147,11 @@ def test_resident_multi_step_launches(name, steps):
     ("gs_2048_wave", "gs_2048", 31, True),            # the same at full size (strips overlap at the last column)
     ("gs_64_tile_pair", "gs_64", 39, True),           # two adjacent points per work-item, 4 steps per launch
     ("gs_2048_tile_pair", "gs_2048", 41, True),       # the same at full size, 32 x 32 tiles
+    ("gs_64_tile_spec", "gs_64", 37, True),           # divisions: whole point again for a wave out of range
+    ("gs_64_tile_guard", "gs_64", 37, True),          # divisions: a per-lane branch each
+    ("gs_64_tile_colmap", "gs_64", 43, True),         # a fixed column per work-item
+    ("gs_64_tile_k5_inplace_colmap", "gs_64", 45, True),
+    ("gs_64_tile_vpair", "gs_64", 39, True),          # two rows of one column per work-item
 ])
 def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     seed = (lambda lib: inputs.heat_field(13, lib.grid)) if name.startswith("heat") else _gs_seed
@@ -118,6 +165,7 @@ def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     ("gs_72x120_tile", "gs_72x120", 41, True),              # 24 x 40 tiles, 3 steps per launch + remainder
     ("gs_72x120_wave", "gs_72x120", 29, True),              # register windows on a 72-row grid
     ("gs_72x120_tile_pair", "gs_72x120", 35, True),         # 24 x 40 tiles, two points per work-item
+    ("gs_72x120_tile_colmap", "gs_72x120", 35, True),       # 24 x 40 tiles, a fixed column per work-item
     ("heat3d_24x40x56", "heat3d_24x40x56", 13, False),      # default (flat) schedule
     ("heat3d_24x40x56_stream", "heat3d_24x40x56", 15, True),  # 8 x 8 columns along a 24-plane axis
     ("heat3d_24x40x56_tb", "heat3d_24x40x56_tb", 5, False),  # 2 steps per Forward, blocks 7 x 11 x 6
