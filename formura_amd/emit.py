@@ -33,6 +33,7 @@ from __future__ import annotations
 
 import json
 import math
+import re
 import struct
 from fractions import Fraction
 from typing import Callable, Dict, List, Optional
@@ -625,6 +626,10 @@ def lib_description(om, name: str, schedule: str) -> dict:
         "filter_interval": cfg.filter_interval,
         "schedule": schedule,
     }
+
+
+# a point function's LDS load statement as Body emits it ("const T name = ldsI[...];")
+LDS_LOAD_LINE = re.compile(r"^const \w+ \w+ = lds\d+\[[^\]]*\];$")
 
 
 def tile_load(count: str, index_lines: List[str], pairs: List[tuple], ctype: str, tag: str = "ld",

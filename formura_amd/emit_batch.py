@@ -42,7 +42,7 @@ import json
 from typing import Dict, List
 
 from . import syntax as S
-from .emit import Body, tile_load, EmitError, GraphPlan, state_fields
+from .emit import LDS_LOAD_LINE, Body, tile_load, EmitError, GraphPlan, state_fields
 from .manifest import calc_range
 
 PRECISIONS = {"f64": ("double", False), "f32k": ("float", True), "f32c": ("float", True),
@@ -281,6 +281,7 @@ class BatchEmitter:
             lines += ["float Ph[NP], Pl[NP];",
                       "#pragma unroll",
                       "for (int q = 0; q < NP; ++q) { Ph[q] = (float)P[q]; Pl[q] = (float)(P[q] - (double)Ph[q]); }"]
+        hoisted = []
         for k in self.plan.stores:
             node = self.plan.mm[k]
             base = self.plan.rng[k].offset - 2 * s
@@ -290,10 +291,14 @@ class BatchEmitter:
             def load(field, c, base=base):
                 return "lds%d[o + (%d)*RY + (%d)]" % (self.fi[field], c[0] + base, c[1] + base)
 
+            # FP64: divisions by constants as the fused sequence, redone with the
+            # guarded form only by a wave holding a dividend outside its range
             body = Body(node, load, lambda *a: None, lambda ax: None, lambda n, v: "", "s%d_" % k,
-                        params=pmap, real=("float" if real == "float" else None))
+                        params=pmap, real=("float" if real == "float" else None), divfix=real != "float")
+            # the point's loads first (names unique to the node), then its arithmetic
+            hoisted += [l for l in body.lines if LDS_LOAD_LINE.search(l)]
             lines.append("{")
-            lines += ["  " + l for l in body.lines]
+            lines += ["  " + l for l in body.lines if not LDS_LOAD_LINE.search(l)]
             if self.compensated:
                 q, dq = self._update_ids(node, fname, base)
 
@@ -373,7 +378,7 @@ class BatchEmitter:
         return TEMPLATE % dict(
             name=self.name, real=real, nf=nf, nslots=nslots, np=max(1, len(self.params)), sleeve=s,
             TX=self.tx, TY=self.ty, NTH=self.threads, KMAX=self.k, G=self.G, GRID=self.grid_wgs, desc=json.dumps(desc),
-            pt_params=params, pt_outs=outs, pt_body="\n".join("  " + l for l in lines),
+            pt_params=params, pt_outs=outs, pt_body="\n".join("  " + l for l in hoisted + lines),
             lds=lds, load_tile=load_tile, srcs=srcs, decl=decl, refs=refs, upd=upd, st_lds=st_lds,
             st_glb=st_glb, CB=cbase, kahan=1 if self.kahan else 0,
             preload="\n".join("  { hipFuncAttributes fa; FMR_CHECK(hipFuncGetAttributes(&fa, (const void*)k_batch<%d>)); }"

@@ -21,11 +21,10 @@ Kernel schedules (chosen per program, see ``choose_schedule``):
 from __future__ import annotations
 
 import json
-import re
 from typing import Dict, List
 
 from . import syntax as S
-from .emit import (tile_load, mixed_precision, mixed_precision_graph, Body, EmitError, GraphPlan, c_elem_type, emit_header,
+from .emit import (LDS_LOAD_LINE, tile_load, mixed_precision, mixed_precision_graph, Body, EmitError, GraphPlan, c_elem_type, emit_header,
                    lib_description, navi_fields, show_double, state_fields, CTYPE)
 from .manifest import MMNode, calc_range
 
@@ -501,9 +500,6 @@ extern "C" long long Formura_HIP_Positivity_Violations(void) { return fmr_rt::vi
 TEMP_GROUP_MICRO = 20000  # instructions of one merged temporary kernel (before sharing): one kernel per level in practice
 
 
-_LDS_LOAD = re.compile(r"^const \w+ \w+ = lds\d+\[[^\]]*\];$")
-
-
 def _indent(text: str, pre: str) -> str:
     return "\n".join(pre + l if l.strip() else l for l in text.split("\n"))
 
@@ -660,9 +656,9 @@ class Tile2DSchedule:
                          divfix=self.div == "fix" and spec_ok is None)
                 used |= b.spec_used
                 # the node's loads (names unique to the node) before every node's arithmetic
-                loads += [l for l in b.lines if _LDS_LOAD.search(l)]
+                loads += [l for l in b.lines if LDS_LOAD_LINE.search(l)]
                 lines.append("{")
-                lines += ["  " + l for l in b.lines if not _LDS_LOAD.search(l)]
+                lines += ["  " + l for l in b.lines if not LDS_LOAD_LINE.search(l)]
                 lines.append("}")
             return loads + lines, used
 
