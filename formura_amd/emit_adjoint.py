@@ -35,6 +35,7 @@ from typing import Dict, List
 from . import syntax as S
 from .emit import Body, EmitError, tile_load, GraphPlan, _const_literal, _f32, show_double, state_fields
 from .manifest import calc_range, inline_temporaries
+from .symadj import SymbolicAdjoint, Unsupported
 
 
 def _operands(inst):
@@ -148,8 +149,10 @@ class ReverseSweep:
 
 class AdjointEmitter:
     def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256,
-                 tile=(32, 32, 4), adj_tile=(16, 64), levels=4, row_block=1, fwd_threads=512, adj_threads=512):
+                 tile=(32, 32, 4), adj_tile=(16, 64), levels=4, row_block=1, fwd_threads=512, adj_threads=512,
+                 symbolic: bool = True):
         self.mm, self.om, self.name = mm, mm.om, name
+        self.symbolic = bool(symbolic)
         self.real = real
         self.tile = tuple(tile) if tile else None
         # tile of the adjoint kernel (defaults to the forward kernel's)
@@ -245,10 +248,58 @@ class AdjointEmitter:
 
     def point_body(self, xload, lload, pguard=None):
         """Statements computing acc<g> (dL/d input field g at q) and pg<p> (the
-        parameter gradients of the output at q) for one point q; returns
+        parameter-gradient contribution of point q) for one point q; returns
         (lines, uses_state): uses_state is False when no derivative reads a
         state-dependent forward value (a step linear in the state with no
-        parameters: its adjoint does not need the forward states)."""
+        parameters: its adjoint does not need the forward states).
+
+        With ``symbolic`` (the default) the body comes from
+        ``symadj.SymbolicAdjoint`` where the step allows it (grouped transposed
+        stencil, parameter gradients summed by parts); otherwise, and with
+        ``symbolic`` off, from the reverse sweep of the instruction list per
+        load offset (``sweep_point_body``)."""
+        sa = self.symbolic_adjoint()
+        if sa is not None:
+            try:
+                nf, np_ = len(self.fields), max(1, len(self.params))
+                lines, uses = sa.point_body(xload, lload, ["acc%d" % g for g in range(nf)],
+                                            ["pg%d" % p for p in range(len(self.params))])
+                if not self.params:
+                    lines.append("double pg0 = 0;")
+                return lines, uses
+            except Unsupported:
+                pass
+        return self.sweep_point_body(xload, lload, pguard)
+
+    def symbolic_adjoint(self):
+        """The step's ``SymbolicAdjoint``, or None (``symbolic`` off, or a step
+        it does not handle)."""
+        if not self.symbolic:
+            return None
+        if not hasattr(self, "_sym"):
+            try:
+                self._sym = SymbolicAdjoint(self.plan, self.dim, self.fields, self.params, self.real)
+            except Unsupported:
+                self._sym = None
+        return self._sym
+
+    def pointwise_state(self) -> bool:
+        """True when the adjoint body reads the state at the point only (or not
+        at all): the tiled adjoint then keeps each level's state in registers
+        instead of staging a state tile with halo in LDS."""
+        sa = self.symbolic_adjoint()
+        if sa is None:
+            return False
+        try:
+            return sa.state_offsets() <= {tuple([0] * self.dim)}
+        except Unsupported:
+            return False
+
+    def sweep_point_body(self, xload, lload, pguard=None):
+        """``point_body`` by the reverse sweep of the step's instruction list:
+        for each load offset delta, the forward instructions at output q - delta
+        and the sweep restricted to the paths from the loads at delta; the
+        parameter gradients at output q (under ``pguard``)."""
         import re
         s = self.plan.sleeve
         dim = self.dim
@@ -442,6 +493,66 @@ class AdjointEmitter:
         st_glb = " ".join("FLD%d(dst)[g] = o%d;" % (i, i) for i in range(nf))
         src.append(TILE_FWD % dict(TX=tx, TY=ty, FTH=self.fwd_threads, lds=lds, load_tile=load_tile, srcs=srcs, decl=decl, refs=refs,
                                    st_lds=st_lds, st_lvl=st_lvl, st_glb=st_glb))
+        if self.pointwise_state():
+            src.append(self._emit_adj_pointwise(uses_state))
+        else:
+            src.append(self._emit_adj_staged(uses_state))
+        fcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, false>), dim3(NT), dim3(%d), 0, s, in, out, P); break;\n"
+                         % (k, k, self.fwd_threads) for k in range(1, kmax + 1))
+        lcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, true>), dim3(NT), dim3(%d), 0, s, in, out, P); break;\n"
+                         % (k, k, self.fwd_threads) for k in range(1, self.levels))
+        acases = "".join("    case %d: hipLaunchKernelGGL(k_adjoint_levels<%d>, dim3(NTA), dim3(%d), 0, s, st0, lv, lam_next, "
+                         "lam_prev, P, pgrad); break;\n" % (k, k, self.adj_threads) for k in range(1, self.levels + 1))
+        src.append(TILE_HOST % dict(TX=tx, TY=ty, AX=ax, AY=ay, K=kmax, fcases=fcases, lcases=lcases, acases=acases))
+        return "\n".join(src)
+
+    def _emit_adj_pointwise(self, uses_state):
+        """k_adjoint_levels<K> for a body that reads the state at the point only:
+        lam in LDS (updated in place per level), each work-item's points fixed
+        for the launch, the state of level m at those points in registers,
+        level m-1's fetched while level m computes."""
+        R = self.real
+        nf = len(self.fields)
+        np_ = max(1, len(self.params))
+        ax, ay = self.adj_tile
+        pb, _ = self.point_body(lambda f, a: "x%d" % f, lambda f, a: "L%d[lo + (%d)*LW + (%d)]" % (f, a[0], a[1]))
+        xs = ", ".join("const real_t x%d" % i for i in range(nf))
+        ls = ", ".join("const real_t* __restrict__ L%d" % i for i in range(nf))
+        aouts = ", ".join(["real_t& oacc%d" % i for i in range(nf)] + ["double& opg%d" % p for p in range(np_)])
+        fin = ["oacc%d = acc%d;" % (i, i) for i in range(nf)] + ["opg%d = pg%d;" % (p, p) for p in range(np_)]
+        out = ["template <int LW>\n__device__ __forceinline__ void adj_point(%s, %s, const int lo, "
+               "const double* __restrict__ P, %s) {\n"
+               "#pragma clang fp reassociate(on)\n#pragma clang fp contract(fast)\n%s\n}\n"
+               % (xs, ls, aouts, "\n".join("  " + l for l in pb + fin))]
+        ll = "\n".join("  __shared__ real_t L%d[LH * LW];" % i for i in range(nf))
+        out.append(TILE_ADJ_PW % dict(
+            TX=ax, TY=ay, ATH=self.adj_threads, ll=ll, uses_state="true" if uses_state else "false",
+            lload=tile_load("LH * LW", ["const int r = idx / LW, c = idx - r * LW;",
+                                        "const long long g = (long long)wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);"],
+                            [("L%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ll",
+                            threads=self.adj_threads),
+            xdecl=" ".join("real_t xv%d[NL_IT], nx%d[NL_IT];" % (i, i) for i in range(nf)),
+            xfetch="\n".join("          nx%d[it] = FLD%d(xs)[gq[it]];" % (i, i) for i in range(nf)),
+            xswap=" ".join("xv%d[it] = nx%d[it];" % (i, i) for i in range(nf)),
+            xargs=", ".join("xv%d[it]" % i for i in range(nf)), largs=", ".join("L%d" % i for i in range(nf)),
+            rdecl=" ".join("real_t res%d[NL_IT];" % i for i in range(nf)),
+            rset=" ".join("res%d[it] = a%d;" % (i, i) for i in range(nf)),
+            adecl=" ".join("real_t a%d;" % i for i in range(nf)) + " " + " ".join("double q%d;" % p for p in range(np_)),
+            aargs=", ".join(["a%d" % i for i in range(nf)] + ["q%d" % p for p in range(np_)]),
+            st_lds=" ".join("L%d[idx] = res%d[it];" % (i, i) for i in range(nf)),
+            st_glb=" ".join("LAM%d(lam_prev)[g] = a%d;" % (i, i) for i in range(nf)),
+            pdecl=" ".join("double pgs%d = 0;" % p for p in range(np_)),
+            pacc=" ".join("pgs%d += q%d;" % (p, p) for p in range(np_)),
+            pgs=", ".join("pgs%d" % p for p in range(np_))))
+        return "\n".join(out)
+
+    def _emit_adj_staged(self, uses_state):
+        """k_adjoint_levels<K> with each level's state tile (and halo) staged in LDS."""
+        R = self.real
+        nf = len(self.fields)
+        np_ = max(1, len(self.params))
+        ax, ay = self.adj_tile
+        src = []
         # adjoint point function on LDS tiles
         pb, _ = self.point_body(lambda f, a: "X%d[xo + (%d)*XW + (%d)]" % (f, a[0], a[1]),
                                 lambda f, a: "L%d[lo + (%d)*LW + (%d)]" % (f, a[0], a[1]), pguard="own")
@@ -477,13 +588,6 @@ class AdjointEmitter:
             pdecl=" ".join("double pgs%d = 0;" % p for p in range(np_)),
             pacc=" ".join("pgs%d += q%d;" % (p, p) for p in range(np_)),
             pgs=", ".join("pgs%d" % p for p in range(np_))))
-        fcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, false>), dim3(NT), dim3(%d), 0, s, in, out, P); break;\n"
-                         % (k, k, self.fwd_threads) for k in range(1, kmax + 1))
-        lcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, true>), dim3(NT), dim3(%d), 0, s, in, out, P); break;\n"
-                         % (k, k, self.fwd_threads) for k in range(1, self.levels))
-        acases = "".join("    case %d: hipLaunchKernelGGL(k_adjoint_levels<%d>, dim3(NTA), dim3(%d), 0, s, st0, lv, lam_next, "
-                         "lam_prev, P, pgrad); break;\n" % (k, k, self.adj_threads) for k in range(1, self.levels + 1))
-        src.append(TILE_HOST % dict(TX=tx, TY=ty, AX=ax, AY=ay, K=kmax, fcases=fcases, lcases=lcases, acases=acases))
         return "\n".join(src)
 
 
@@ -652,6 +756,93 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_levels(const char* __restri
       if (USES_STATE) {
         xput(m - 1);
         if (m > 1) xfetch(m - 2);
+      }
+      __syncthreads();
+    }
+  }
+  { const double pv[NP] = {%(pgs)s}; block_reduce_params<%(ATH)d>(pgrad + (long long)blockIdx.x * NP, pv); }
+}
+"""
+
+TILE_ADJ_PW = r"""
+// K adjoint steps: lam_K (HBM) -> lam_0 (HBM); X_0 = st0, X_m = level buffer m-1 of lv.
+// The adjoint body reads the state at the point only: work-item points are fixed
+// for the launch (element it of this thread is (r, c) = divmod(idx, LW) of the
+// lam buffer), the state of level m at those points is in registers (xv*), and
+// level m-1's is fetched (nx*) while level m computes.
+template <int K>
+__global__ void __launch_bounds__(%(ATH)d) k_adjoint_levels(const char* __restrict__ st0, const char* __restrict__ lv,
+                                                        const char* __restrict__ lam_next, char* __restrict__ lam_prev,
+                                                        const double* __restrict__ P, double* __restrict__ pgrad) {
+  constexpr int TX = %(TX)d, TY = %(TY)d;
+  constexpr int LH = TX + 2 * K * SLEEVE, LW = TY + 2 * K * SLEEVE;
+  constexpr int NTY = N2 / TY, NT = (N1 / TX) * NTY;
+  constexpr bool USES_STATE = %(uses_state)s;
+  static_assert((long long)N1 * N2 < (1LL << 31), "element offsets of a field are ints");
+%(ll)s
+  const int tile = xcd_tile(blockIdx.x, NT);
+  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+  double Pl[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) Pl[p] = HAS_PARAMS ? P[p] : 0.0;
+  constexpr int NL_IT = (LH * LW + %(ATH)d - 1) / %(ATH)d;
+  // element it: row rq[it], column cq[it] of the lam buffer, its periodic global offset gq[it]
+  int rq[NL_IT], cq[NL_IT], gq[NL_IT];
+#pragma unroll
+  for (int it = 0; it < NL_IT; ++it) {
+    const int idx = threadIdx.x + it * %(ATH)d;
+    const int r = idx / LW, c = idx - r * LW;
+    rq[it] = idx < LH * LW ? r : LH;
+    cq[it] = c;
+    gq[it] = wrapn(x0 + (idx < LH * LW ? r : 0), N1) * N2 + wrapn(y0 + c, N2);
+  }
+  %(xdecl)s
+  // the state of level m at the elements level m computes (rows < TX + 2ms, columns < TY + 2ms)
+  auto xfetch = [&](int m) {
+    const char* __restrict__ xs = m == 0 ? st0 : lv + (size_t)(m - 1) * STATE_BYTES;
+#pragma unroll
+    for (int it = 0; it < NL_IT; ++it) {
+      if (rq[it] < TX + 2 * m * SLEEVE && cq[it] < TY + 2 * m * SLEEVE) {
+%(xfetch)s
+      }
+    }
+  };
+  if (USES_STATE) xfetch(K - 1);
+%(lload)s
+  __syncthreads();
+  %(pdecl)s
+#pragma unroll
+  for (int mm = 0; mm < K; ++mm) {
+    const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} -> lam_m at state X_m
+    const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
+    if (USES_STATE) {
+#pragma unroll
+      for (int it = 0; it < NL_IT; ++it) { %(xswap)s }
+      if (m > 0) xfetch(m - 1);
+    }
+    %(rdecl)s
+#pragma unroll
+    for (int it = 0; it < NL_IT; ++it) {
+      const int r = rq[it], c = cq[it];
+      if (r < HH && c < W) {
+        const int lo = r * LW + c;
+        %(adecl)s
+        adj_point<LW>(%(xargs)s, %(largs)s, lo, Pl, %(aargs)s);
+        if (m > 0) {
+          %(rset)s
+        } else {
+          const long long g = (long long)(x0 + r) * N2 + (y0 + c);
+          %(st_glb)s
+        }
+        if (r < TX && c < TY) { %(pacc)s }
+      }
+    }
+    if (m > 0) {
+      __syncthreads();  // every read of lam_{m+1} is done
+#pragma unroll
+      for (int it = 0; it < NL_IT; ++it) {
+        const int idx = threadIdx.x + it * %(ATH)d;
+        if (rq[it] < HH && cq[it] < W) { %(st_lds)s }
       }
       __syncthreads();
     }
