@@ -150,9 +150,14 @@ class ReverseSweep:
 class AdjointEmitter:
     def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256,
                  tile=(32, 32, 4), adj_tile=(16, 64), levels=4, row_block=1, fwd_threads=512, adj_threads=512,
-                 symbolic: bool = True):
+                 symbolic: bool = True, fused: bool = True, adj_prefetch: bool = True, adj_index_arrays: bool = True):
         self.mm, self.om, self.name = mm, mm.om, name
         self.symbolic = bool(symbolic)
+        # units of adjoint steps through k_adjoint_unit (odd levels recomputed in the tile)
+        self.fused_ok = bool(fused)
+        # pointwise adjoint kernel: fetch the next level's state while a level
+        # computes (registers), and keep element indices in registers
+        self.adj_prefetch, self.adj_index_arrays = bool(adj_prefetch), bool(adj_index_arrays)
         self.real = real
         self.tile = tuple(tile) if tile else None
         # tile of the adjoint kernel (defaults to the forward kernel's)
@@ -295,6 +300,13 @@ class AdjointEmitter:
         except Unsupported:
             return False
 
+    def fused_unit(self) -> bool:
+        """True when a unit of up to 4 adjoint steps runs as a two-step forward
+        launch (the unit's level-2 state) and one ``k_adjoint_unit`` launch that
+        recomputes each odd level from the even level below it in its tile:
+        tiled 2-D programs whose adjoint reads the state at the point only."""
+        return bool(self.fused_ok and self.tiled() and self.pointwise_state() and self.levels <= 4)
+
     def sweep_point_body(self, xload, lload, pguard=None):
         """``point_body`` by the reverse sweep of the step's instruction list:
         for each load offset delta, the forward instructions at output q - delta
@@ -414,6 +426,10 @@ class AdjointEmitter:
             steps_call = ("  if (nsteps > 1) fwd_levels(nsteps - 1, (const char*)state0, (char*)scratch, params_dev, s);\n"
                           "  adj_levels(nsteps, (const char*)state0, (const char*)scratch, (const char*)lam_next, "
                           "(char*)lam_prev, params_dev, pgrad, s);")
+            if self.fused_unit():
+                steps_call = ("  if (nsteps > 2) fwd_even((const char*)state0, (char*)scratch, params_dev, s);\n"
+                              "  adj_unit(nsteps, (const char*)state0, (const char*)scratch, (const char*)lam_next, "
+                              "(char*)lam_prev, params_dev, pgrad, s);")
             inner_call = "  if (nlevels > 0) fwd_levels(nlevels, (const char*)state0, (char*)scratch, params_dev, s);"
             levels_call = ("  adj_levels(nsteps, (const char*)state0, (const char*)scratch, (const char*)lam_next, "
                            "(char*)lam_prev, params_dev, pgrad, s);")
@@ -434,12 +450,15 @@ class AdjointEmitter:
             slots = None
             levels = 1
         desc["levels"] = levels
+        desc["fused_unit"] = bool(tile and self.fused_unit())
         return ADJ_TEMPLATE % dict(
             name=self.name, R=R, nf=nf, np=np_, has_params="true" if self.params else "false",
             block=self.block, sleeve=s,
             consts="\n".join("constexpr int N%d = %d;" % (k + 1, n) for k, n in enumerate(self.N)),
             ntot="*".join("(long long)%d" % n for n in self.N), flds=flds, lams=lams,
             desc=json.dumps(desc, sort_keys=True), fwd="\n".join("  " + l for l in fwd), adj="\n".join("  " + l for l in body),
+            fused="true" if (tile and self.fused_unit()) else "false",
+            unit_stub="" if (tile and self.fused_unit()) else UNIT_STUB,
             tiled=tiled_src, fwd_call=fwd_call, adj_call=adj_call, steps_call=steps_call, levels=levels,
             inner_call=inner_call, levels_call=levels_call,
             slots=("%d" % slots) if slots else "(int)((NTOT + BLOCK - 1) / BLOCK)")
@@ -497,6 +516,12 @@ class AdjointEmitter:
             src.append(self._emit_adj_pointwise(uses_state))
         else:
             src.append(self._emit_adj_staged(uses_state))
+        ucases = ""
+        if self.fused_unit():
+            src.append(self._emit_adj_unit(uses_state))
+            ucases = "".join("    case %d: hipLaunchKernelGGL(k_adjoint_unit<%d>, dim3(NTA), dim3(%d), 0, s, st0, ev, lam_next, "
+                             "lam_prev, P, pgrad); break;\n" % (k, k, self.adj_threads)
+                             for k in range(1, self.levels + 1))
         fcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, false>), dim3(NT), dim3(%d), 0, s, in, out, P); break;\n"
                          % (k, k, self.fwd_threads) for k in range(1, kmax + 1))
         lcases = "".join("    case %d: hipLaunchKernelGGL((k_forward_tile<%d, true>), dim3(NT), dim3(%d), 0, s, in, out, P); break;\n"
@@ -504,7 +529,50 @@ class AdjointEmitter:
         acases = "".join("    case %d: hipLaunchKernelGGL(k_adjoint_levels<%d>, dim3(NTA), dim3(%d), 0, s, st0, lv, lam_next, "
                          "lam_prev, P, pgrad); break;\n" % (k, k, self.adj_threads) for k in range(1, self.levels + 1))
         src.append(TILE_HOST % dict(TX=tx, TY=ty, AX=ax, AY=ay, K=kmax, fcases=fcases, lcases=lcases, acases=acases))
+        if ucases:
+            src.append(UNIT_HOST % dict(ucases=ucases))
         return "\n".join(src)
+
+    def _emit_adj_unit(self, uses_state):
+        """k_adjoint_unit<K>: K (<= 4) adjoint levels over the adjoint tile; the
+        state of an odd level m is recomputed at the points from level m-1
+        (st0 or the level-2 state ev) staged in LDS with its low-side halo (the
+        step's exact instruction list, as the forward kernels evaluate it), an
+        even level's is read from that frame (or from HBM when no odd level
+        above it staged it)."""
+        nf = len(self.fields)
+        np_ = max(1, len(self.params))
+        ax, ay = self.adj_tile
+        th = self.adj_threads
+        fl = "\n".join("  __shared__ real_t F%d[LH * LW];" % i for i in range(nf))
+        ll = "\n".join("  __shared__ real_t L%d[LH * LW];" % i for i in range(nf))
+        stage = tile_load("FR * LW", ["const int r = idx / LW, c = idx - r * LW;",
+                                      "const long long g = (long long)wrapn(x0 - 2 * SLEEVE + r, N1) * N2 + "
+                                      "wrapn(y0 - 2 * SLEEVE + c, N2);"],
+                          [("F%d[idx]" % i, "FLD%d(xs)[g]" % i) for i in range(nf)], "real_t", "fs",
+                          threads=th, indent="        ", max_count="LH * LW")
+        return TILE_ADJ_UNIT % dict(
+            TX=ax, TY=ay, ATH=th, fl=fl, ll=ll, stage=stage,
+            idxarr="true" if self.adj_index_arrays else "false",
+            lload=tile_load("LH * LW", ["const int r = idx / LW, c = idx - r * LW;",
+                                        "const long long g = (long long)wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);"],
+                            [("L%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ul",
+                            threads=th),
+            srcs=", ".join("F%d" % i for i in range(nf)),
+            xdecl=" ".join("real_t sv%d;" % i for i in range(nf)),
+            xstep=", ".join("sv%d" % i for i in range(nf)),
+            xframe=" ".join("sv%d = F%d[fo];" % (i, i) for i in range(nf)),
+            xglob=" ".join("sv%d = FLD%d(xs)[g];" % (i, i) for i in range(nf)),
+            xargs=", ".join("sv%d" % i for i in range(nf)), largs=", ".join("L%d" % i for i in range(nf)),
+            rdecl=" ".join("real_t res%d[NL_IT];" % i for i in range(nf)),
+            rset=" ".join("res%d[it] = a%d;" % (i, i) for i in range(nf)),
+            adecl=" ".join("real_t a%d;" % i for i in range(nf)) + " " + " ".join("double q%d;" % p for p in range(np_)),
+            aargs=", ".join(["a%d" % i for i in range(nf)] + ["q%d" % p for p in range(np_)]),
+            st_lds=" ".join("L%d[lo] = res%d[it];" % (i, i) for i in range(nf)),
+            st_glb=" ".join("LAM%d(lam_prev)[go] = a%d;" % (i, i) for i in range(nf)),
+            pdecl=" ".join("double pgs%d = 0;" % p for p in range(np_)),
+            pacc=" ".join("pgs%d += q%d;" % (p, p) for p in range(np_)),
+            pgs=", ".join("pgs%d" % p for p in range(np_)))
 
     def _emit_adj_pointwise(self, uses_state):
         """k_adjoint_levels<K> for a body that reads the state at the point only:
@@ -522,7 +590,7 @@ class AdjointEmitter:
         fin = ["oacc%d = acc%d;" % (i, i) for i in range(nf)] + ["opg%d = pg%d;" % (p, p) for p in range(np_)]
         out = ["template <int LW>\n__device__ __forceinline__ void adj_point(%s, %s, const int lo, "
                "const double* __restrict__ P, %s) {\n"
-               "#pragma clang fp reassociate(on)\n#pragma clang fp contract(fast)\n%s\n}\n"
+               "#pragma clang fp contract(fast)\n%s\n}\n"
                % (xs, ls, aouts, "\n".join("  " + l for l in pb + fin))]
         ll = "\n".join("  __shared__ real_t L%d[LH * LW];" % i for i in range(nf))
         out.append(TILE_ADJ_PW % dict(
@@ -531,16 +599,18 @@ class AdjointEmitter:
                                         "const long long g = (long long)wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);"],
                             [("L%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ll",
                             threads=self.adj_threads),
-            xdecl=" ".join("real_t xv%d[NL_IT], nx%d[NL_IT];" % (i, i) for i in range(nf)),
-            xfetch="\n".join("          nx%d[it] = FLD%d(xs)[gq[it]];" % (i, i) for i in range(nf)),
+            prefetch="true" if self.adj_prefetch else "false", idxarr="true" if self.adj_index_arrays else "false",
+            xdecl=" ".join("real_t xv%d[NL_IT], nx%d[PREFETCH ? NL_IT : 1];" % (i, i) for i in range(nf)),
+            xfetch_nx="\n".join("          nx%d[it] = FLD%d(xs)[g];" % (i, i) for i in range(nf)),
+            xfetch_xv="\n".join("          xv%d[it] = FLD%d(xs)[g];" % (i, i) for i in range(nf)),
             xswap=" ".join("xv%d[it] = nx%d[it];" % (i, i) for i in range(nf)),
             xargs=", ".join("xv%d[it]" % i for i in range(nf)), largs=", ".join("L%d" % i for i in range(nf)),
             rdecl=" ".join("real_t res%d[NL_IT];" % i for i in range(nf)),
             rset=" ".join("res%d[it] = a%d;" % (i, i) for i in range(nf)),
             adecl=" ".join("real_t a%d;" % i for i in range(nf)) + " " + " ".join("double q%d;" % p for p in range(np_)),
             aargs=", ".join(["a%d" % i for i in range(nf)] + ["q%d" % p for p in range(np_)]),
-            st_lds=" ".join("L%d[idx] = res%d[it];" % (i, i) for i in range(nf)),
-            st_glb=" ".join("LAM%d(lam_prev)[g] = a%d;" % (i, i) for i in range(nf)),
+            st_lds=" ".join("L%d[lo] = res%d[it];" % (i, i) for i in range(nf)),
+            st_glb=" ".join("LAM%d(lam_prev)[go] = a%d;" % (i, i) for i in range(nf)),
             pdecl=" ".join("double pgs%d = 0;" % p for p in range(np_)),
             pacc=" ".join("pgs%d += q%d;" % (p, p) for p in range(np_)),
             pgs=", ".join("pgs%d" % p for p in range(np_))))
@@ -768,8 +838,10 @@ TILE_ADJ_PW = r"""
 // K adjoint steps: lam_K (HBM) -> lam_0 (HBM); X_0 = st0, X_m = level buffer m-1 of lv.
 // The adjoint body reads the state at the point only: work-item points are fixed
 // for the launch (element it of this thread is (r, c) = divmod(idx, LW) of the
-// lam buffer), the state of level m at those points is in registers (xv*), and
-// level m-1's is fetched (nx*) while level m computes.
+// lam buffer) and the state of level m at those points is in registers (xv*);
+// with PREFETCH level m-1's is fetched (nx*) while level m computes.  With
+// IDXARR the elements' rows, columns and global offsets stay in registers,
+// else they are recomputed where used (fewer registers, more waves per CU).
 template <int K>
 __global__ void __launch_bounds__(%(ATH)d) k_adjoint_levels(const char* __restrict__ st0, const char* __restrict__ lv,
                                                         const char* __restrict__ lam_next, char* __restrict__ lam_prev,
@@ -777,7 +849,7 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_levels(const char* __restri
   constexpr int TX = %(TX)d, TY = %(TY)d;
   constexpr int LH = TX + 2 * K * SLEEVE, LW = TY + 2 * K * SLEEVE;
   constexpr int NTY = N2 / TY, NT = (N1 / TX) * NTY;
-  constexpr bool USES_STATE = %(uses_state)s;
+  constexpr bool USES_STATE = %(uses_state)s, PREFETCH = %(prefetch)s, IDXARR = %(idxarr)s;
   static_assert((long long)N1 * N2 < (1LL << 31), "element offsets of a field are ints");
 %(ll)s
   const int tile = xcd_tile(blockIdx.x, NT);
@@ -786,28 +858,48 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_levels(const char* __restri
 #pragma unroll
   for (int p = 0; p < NP; ++p) Pl[p] = HAS_PARAMS ? P[p] : 0.0;
   constexpr int NL_IT = (LH * LW + %(ATH)d - 1) / %(ATH)d;
-  // element it: row rq[it], column cq[it] of the lam buffer, its periodic global offset gq[it]
-  int rq[NL_IT], cq[NL_IT], gq[NL_IT];
+  // element it: row (LH: none) and column of the lam buffer, periodic global offset
+  int rq[IDXARR ? NL_IT : 1], cq[IDXARR ? NL_IT : 1], gq[IDXARR ? NL_IT : 1];
+  auto element = [&](int it, int& r, int& c, int& g) {
+    if constexpr (IDXARR) {
+      r = rq[it]; c = cq[it]; g = gq[it];
+    } else {
+      const int idx = threadIdx.x + it * %(ATH)d;
+      const int rr = idx / LW;
+      c = idx - rr * LW;
+      r = idx < LH * LW ? rr : LH;
+      g = wrapn(x0 + (idx < LH * LW ? rr : 0), N1) * N2 + wrapn(y0 + c, N2);
+    }
+  };
+  if constexpr (IDXARR) {
 #pragma unroll
-  for (int it = 0; it < NL_IT; ++it) {
-    const int idx = threadIdx.x + it * %(ATH)d;
-    const int r = idx / LW, c = idx - r * LW;
-    rq[it] = idx < LH * LW ? r : LH;
-    cq[it] = c;
-    gq[it] = wrapn(x0 + (idx < LH * LW ? r : 0), N1) * N2 + wrapn(y0 + c, N2);
+    for (int it = 0; it < NL_IT; ++it) {
+      const int idx = threadIdx.x + it * %(ATH)d;
+      const int r = idx / LW, c = idx - r * LW;
+      rq[it] = idx < LH * LW ? r : LH;
+      cq[it] = c;
+      gq[it] = wrapn(x0 + (idx < LH * LW ? r : 0), N1) * N2 + wrapn(y0 + c, N2);
+    }
   }
   %(xdecl)s
-  // the state of level m at the elements level m computes (rows < TX + 2ms, columns < TY + 2ms)
+  // the state of level m at the elements level m computes (rows < TX + 2ms,
+  // columns < TY + 2ms), into nx (PREFETCH) or xv
   auto xfetch = [&](int m) {
     const char* __restrict__ xs = m == 0 ? st0 : lv + (size_t)(m - 1) * STATE_BYTES;
 #pragma unroll
     for (int it = 0; it < NL_IT; ++it) {
-      if (rq[it] < TX + 2 * m * SLEEVE && cq[it] < TY + 2 * m * SLEEVE) {
-%(xfetch)s
+      int r, c, g;
+      element(it, r, c, g);
+      if (r < TX + 2 * m * SLEEVE && c < TY + 2 * m * SLEEVE) {
+        if constexpr (PREFETCH) {
+%(xfetch_nx)s
+        } else {
+%(xfetch_xv)s
+        }
       }
     }
   };
-  if (USES_STATE) xfetch(K - 1);
+  if (USES_STATE && PREFETCH) xfetch(K - 1);
 %(lload)s
   __syncthreads();
   %(pdecl)s
@@ -816,14 +908,19 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_levels(const char* __restri
     const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} -> lam_m at state X_m
     const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
     if (USES_STATE) {
+      if constexpr (PREFETCH) {
 #pragma unroll
-      for (int it = 0; it < NL_IT; ++it) { %(xswap)s }
-      if (m > 0) xfetch(m - 1);
+        for (int it = 0; it < NL_IT; ++it) { %(xswap)s }
+        if (m > 0) xfetch(m - 1);
+      } else {
+        xfetch(m);
+      }
     }
     %(rdecl)s
 #pragma unroll
     for (int it = 0; it < NL_IT; ++it) {
-      const int r = rq[it], c = cq[it];
+      int r, c, g;
+      element(it, r, c, g);
       if (r < HH && c < W) {
         const int lo = r * LW + c;
         %(adecl)s
@@ -831,7 +928,7 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_levels(const char* __restri
         if (m > 0) {
           %(rset)s
         } else {
-          const long long g = (long long)(x0 + r) * N2 + (y0 + c);
+          const long long go = (long long)(x0 + r) * N2 + (y0 + c);
           %(st_glb)s
         }
         if (r < TX && c < TY) { %(pacc)s }
@@ -841,13 +938,142 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_levels(const char* __restri
       __syncthreads();  // every read of lam_{m+1} is done
 #pragma unroll
       for (int it = 0; it < NL_IT; ++it) {
-        const int idx = threadIdx.x + it * %(ATH)d;
-        if (rq[it] < HH && cq[it] < W) { %(st_lds)s }
+        int r, c, g;
+        element(it, r, c, g);
+        if (r < HH && c < W) { const int lo = r * LW + c; %(st_lds)s }
       }
       __syncthreads();
     }
   }
   { const double pv[NP] = {%(pgs)s}; block_reduce_params<%(ATH)d>(pgrad + (long long)blockIdx.x * NP, pv); }
+}
+"""
+
+TILE_ADJ_UNIT = r"""
+// K (<= 4) adjoint steps from the state at their start (st0) and the state two
+// steps later (ev, for K >= 3): lam_K (HBM) -> lam_0 (HBM).  Levels come in
+// pairs (odd m, even m-1): level m-1's state is staged in the frame F (tile rows
+// [-2s, T + 2ms), the low-side halo the step reads), level m's state is
+// recomputed at each point of the adjoint region from F with the step's exact
+// instruction list, and level m-1's is read from F.  The adjoint body reads the
+// state at the point only; lam is updated in place in LDS.
+template <int K>
+__global__ void __launch_bounds__(%(ATH)d) k_adjoint_unit(const char* __restrict__ st0, const char* __restrict__ ev,
+                                                      const char* __restrict__ lam_next, char* __restrict__ lam_prev,
+                                                      const double* __restrict__ P, double* __restrict__ pgrad) {
+  constexpr int TX = %(TX)d, TY = %(TY)d;
+  constexpr int LH = TX + 2 * K * SLEEVE, LW = TY + 2 * K * SLEEVE;
+  constexpr int NTY = N2 / TY, NT = (N1 / TX) * NTY;
+  constexpr bool IDXARR = %(idxarr)s;
+  static_assert(K <= 4, "levels 0 and 2 are given, 1 and 3 recomputed");
+  static_assert((long long)N1 * N2 < (1LL << 31), "element offsets of a field are ints");
+%(fl)s
+%(ll)s
+  const int tile = xcd_tile(blockIdx.x, NT);
+  const int x0 = (tile / NTY) * TX, y0 = (tile %% NTY) * TY;
+  double Pl[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) Pl[p] = HAS_PARAMS ? P[p] : 0.0;
+  constexpr int NL_IT = (LH * LW + %(ATH)d - 1) / %(ATH)d;
+  int rq[IDXARR ? NL_IT : 1], cq[IDXARR ? NL_IT : 1];
+  auto element = [&](int it, int& r, int& c) {
+    if constexpr (IDXARR) {
+      r = rq[it]; c = cq[it];
+    } else {
+      const int idx = threadIdx.x + it * %(ATH)d;
+      const int rr = idx / LW;
+      c = idx - rr * LW;
+      r = idx < LH * LW ? rr : LH;
+    }
+  };
+  if constexpr (IDXARR) {
+#pragma unroll
+    for (int it = 0; it < NL_IT; ++it) {
+      const int idx = threadIdx.x + it * %(ATH)d;
+      const int r = idx / LW;
+      rq[it] = idx < LH * LW ? r : LH;
+      cq[it] = idx - r * LW;
+    }
+  }
+%(lload)s
+  %(pdecl)s
+#pragma unroll
+  for (int mm = 0; mm < K; ++mm) {
+    const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} -> lam_m at state m
+    const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
+    // state m: recomputed (odd m) from the frame, read from the frame (even m
+    // below an odd level), or read from HBM (even m = K-1)
+    const bool recompute = (m & 1) != 0, framed = !recompute && m + 1 < K;
+    if (recompute) {
+      const char* __restrict__ xs = m == 1 ? st0 : ev;  // level m-1
+      constexpr int FR_MAX = LH;
+      const int FR = TX + 2 * (m + 1) * SLEEVE;  // frame rows staged: [-2s, T + 2ms) of the tile
+      (void)FR_MAX;
+%(stage)s
+    }
+    __syncthreads();  // lam_{m+1} and the frame are in LDS
+    %(rdecl)s
+#pragma unroll
+    for (int it = 0; it < NL_IT; ++it) {
+      int r, c;
+      element(it, r, c);
+      if (r < HH && c < W) {
+        const int lo = r * LW + c, fo = (r + 2 * SLEEVE) * LW + (c + 2 * SLEEVE);
+        %(xdecl)s
+        if (recompute) {
+          pt_step<LW>(%(srcs)s, fo, Pl, %(xstep)s);
+        } else if (framed) {
+          %(xframe)s
+        } else {
+          const char* __restrict__ xs = m == 0 ? st0 : ev;
+          const int g = wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);
+          %(xglob)s
+        }
+        %(adecl)s
+        adj_point<LW>(%(xargs)s, %(largs)s, lo, Pl, %(aargs)s);
+        if (m > 0) {
+          %(rset)s
+        } else {
+          const long long go = (long long)(x0 + r) * N2 + (y0 + c);
+          %(st_glb)s
+        }
+        if (r < TX && c < TY) { %(pacc)s }
+      }
+    }
+    if (m > 0) {
+      __syncthreads();  // every read of lam_{m+1} (and of the frame) is done
+#pragma unroll
+      for (int it = 0; it < NL_IT; ++it) {
+        int r, c;
+        element(it, r, c);
+        if (r < HH && c < W) { const int lo = r * LW + c; %(st_lds)s }
+      }
+    }
+  }
+  { const double pv[NP] = {%(pgs)s}; block_reduce_params<%(ATH)d>(pgrad + (long long)blockIdx.x * NP, pv); }
+}
+"""
+
+UNIT_STUB = r"""
+// no fused unit in this library (FUSED_UNIT is false): never called
+static void fwd_even(const char*, char*, const double*, hipStream_t) { fmr_rt::fatal("fwd_even: no fused unit"); }
+static void adj_unit(int, const char*, const char*, const char*, char*, const double*, double*, hipStream_t) {
+  fmr_rt::fatal("adj_unit: no fused unit");
+}
+"""
+
+UNIT_HOST = r"""
+// the state two steps after `in` into `out` (one launch)
+static void fwd_even(const char* in, char* out, const double* P, hipStream_t s) {
+  fwd_tiles(in, [&](int) { return out; }, 2, P, s);
+}
+
+// the adjoint of k (1..KA <= 4) steps from st0 in one launch (ev: the state two steps later, for k >= 3)
+static void adj_unit(int k, const char* st0, const char* ev, const char* lam_next, char* lam_prev, const double* P,
+                     double* pgrad, hipStream_t s) {
+  switch (k) {
+%(ucases)s    default: fmr_rt::fatal("bad adjoint step count");
+  }
 }
 """
 
@@ -903,6 +1129,8 @@ constexpr int BLOCK = %(block)d;
 constexpr int SLEEVE = %(sleeve)d;
 constexpr int KA = %(levels)d;  // most adjoint steps per Formura_AD_Adjoint_Steps call
 constexpr size_t STATE_BYTES = (size_t)NF * NTOT * sizeof(%(R)s);
+// a unit of adjoint steps is one kernel that recomputes its inner states (no scratch used)
+constexpr bool FUSED_UNIT = %(fused)s;
 %(flds)s
 %(lams)s
 const char* const kDescription = R"FMRJSON(%(desc)s)FMRJSON";
@@ -939,6 +1167,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const char* __restrict__ st, 
 %(adj)s
 }
 %(tiled)s
+%(unit_stub)s
 }  // namespace
 
 extern "C" const char* Formura_AD_Describe(void) { return kDescription; }
@@ -1050,16 +1279,26 @@ extern "C" int Formura_AD_Reverse(const int* ops, int nops, void* const* bufs, i
     if (kind == 0) {  // bufs[b] = k forward steps from bufs[a], on F
       if (two && was_read[b]) FMR_CHECK(hipStreamWaitEvent(F, ev.e[b], 0));
       Formura_AD_Forward(bufs[a], bufs[b], tmp, k, params_dev, (void*)F);
-    } else {  // the adjoint of the k steps from bufs[a], on A (inner states on F)
+    } else {  // the adjoint of the k steps from bufs[a], on A (inner states on F, or in the fused kernel)
       if (k > KA) fmr_rt::fatal("Formura_AD_Reverse: unit longer than Formura_AD_Levels()");
       const int h = KA > 1 ? (turn++ & 1) : 0;
-      if (two && half_used[h]) FMR_CHECK(hipStreamWaitEvent(F, ev.e[nbufs + h], 0));
-      if (k > 1) Formura_AD_Inner_States(bufs[a], sc[h], k - 1, params_dev, (void*)F);
-      if (two) {
-        FMR_CHECK(hipEventRecord(ev.e[nbufs + 2 + h], F));
-        FMR_CHECK(hipStreamWaitEvent(A, ev.e[nbufs + 2 + h], 0));
+      if (FUSED_UNIT) {  // the level-2 state on F, the unit's adjoint (odd levels recomputed) on A
+        if (two && half_used[h]) FMR_CHECK(hipStreamWaitEvent(F, ev.e[nbufs + h], 0));
+        if (k > 2) fwd_even((const char*)bufs[a], sc[h], params_dev, F);
+        if (two) {
+          FMR_CHECK(hipEventRecord(ev.e[nbufs + 2 + h], F));
+          FMR_CHECK(hipStreamWaitEvent(A, ev.e[nbufs + 2 + h], 0));
+        }
+        adj_unit(k, (const char*)bufs[a], sc[h], lcur, lnxt, params_dev, pgrad, A);
+      } else {
+        if (two && half_used[h]) FMR_CHECK(hipStreamWaitEvent(F, ev.e[nbufs + h], 0));
+        if (k > 1) Formura_AD_Inner_States(bufs[a], sc[h], k - 1, params_dev, (void*)F);
+        if (two) {
+          FMR_CHECK(hipEventRecord(ev.e[nbufs + 2 + h], F));
+          FMR_CHECK(hipStreamWaitEvent(A, ev.e[nbufs + 2 + h], 0));
+        }
+        Formura_AD_Adjoint_Levels(bufs[a], sc[h], lcur, lnxt, k, params_dev, pgrad, (void*)A);
       }
-      Formura_AD_Adjoint_Levels(bufs[a], sc[h], lcur, lnxt, k, params_dev, pgrad, (void*)A);
       char* t = lcur;
       lcur = lnxt;
       lnxt = t;
