@@ -149,7 +149,7 @@ class ReverseSweep:
 
 class AdjointEmitter:
     def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256,
-                 tile=(32, 32, 4), adj_tile=(16, 64), levels=4, row_block=1, fwd_threads=512, adj_threads=512,
+                 tile=(32, 32, 4), adj_tile=(32, 32), levels=4, row_block=1, fwd_threads=512, adj_threads=512,
                  symbolic: bool = True, fused: bool = True, adj_prefetch: bool = True, adj_index_arrays: bool = True):
         self.mm, self.om, self.name = mm, mm.om, name
         self.symbolic = bool(symbolic)
@@ -545,7 +545,7 @@ class AdjointEmitter:
         ax, ay = self.adj_tile
         th = self.adj_threads
         fl = "\n".join("  __shared__ real_t F%d[LH * LW];" % i for i in range(nf))
-        ll = "\n".join("  __shared__ real_t L%d[LH * LW];" % i for i in range(nf))
+        ll = "\n".join("  __shared__ real_t L%d[2][LH * LW];" % i for i in range(nf))
         stage = tile_load("FR * LW", ["const int r = idx / LW, c = idx - r * LW;",
                                       "const long long g = (long long)wrapn(x0 - 2 * SLEEVE + r, N1) * N2 + "
                                       "wrapn(y0 - 2 * SLEEVE + c, N2);"],
@@ -553,22 +553,19 @@ class AdjointEmitter:
                           threads=th, indent="        ", max_count="LH * LW")
         return TILE_ADJ_UNIT % dict(
             TX=ax, TY=ay, ATH=th, fl=fl, ll=ll, stage=stage,
-            idxarr="true" if self.adj_index_arrays else "false",
             lload=tile_load("LH * LW", ["const int r = idx / LW, c = idx - r * LW;",
                                         "const long long g = (long long)wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);"],
-                            [("L%d[idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ul",
+                            [("L%d[0][idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ul",
                             threads=th),
             srcs=", ".join("F%d" % i for i in range(nf)),
             xdecl=" ".join("real_t sv%d;" % i for i in range(nf)),
             xstep=", ".join("sv%d" % i for i in range(nf)),
             xframe=" ".join("sv%d = F%d[fo];" % (i, i) for i in range(nf)),
             xglob=" ".join("sv%d = FLD%d(xs)[g];" % (i, i) for i in range(nf)),
-            xargs=", ".join("sv%d" % i for i in range(nf)), largs=", ".join("L%d" % i for i in range(nf)),
-            rdecl=" ".join("real_t res%d[NL_IT];" % i for i in range(nf)),
-            rset=" ".join("res%d[it] = a%d;" % (i, i) for i in range(nf)),
+            xargs=", ".join("sv%d" % i for i in range(nf)), largs=", ".join("L%d[sb]" % i for i in range(nf)),
             adecl=" ".join("real_t a%d;" % i for i in range(nf)) + " " + " ".join("double q%d;" % p for p in range(np_)),
             aargs=", ".join(["a%d" % i for i in range(nf)] + ["q%d" % p for p in range(np_)]),
-            st_lds=" ".join("L%d[lo] = res%d[it];" % (i, i) for i in range(nf)),
+            st_lds=" ".join("L%d[db][lo] = a%d;" % (i, i) for i in range(nf)),
             st_glb=" ".join("LAM%d(lam_prev)[go] = a%d;" % (i, i) for i in range(nf)),
             pdecl=" ".join("double pgs%d = 0;" % p for p in range(np_)),
             pacc=" ".join("pgs%d += q%d;" % (p, p) for p in range(np_)),
@@ -956,7 +953,9 @@ TILE_ADJ_UNIT = r"""
 // [-2s, T + 2ms), the low-side halo the step reads), level m's state is
 // recomputed at each point of the adjoint region from F with the step's exact
 // instruction list, and level m-1's is read from F.  The adjoint body reads the
-// state at the point only; lam is updated in place in LDS.
+// state at the point only.  lam ping-pongs between two LDS buffers (one barrier
+// per level); level m's work-items cover exactly its region [0, TX + 2ms) x
+// [0, TY + 2ms).
 template <int K>
 __global__ void __launch_bounds__(%(ATH)d) k_adjoint_unit(const char* __restrict__ st0, const char* __restrict__ ev,
                                                       const char* __restrict__ lam_next, char* __restrict__ lam_prev,
@@ -964,7 +963,6 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_unit(const char* __restrict
   constexpr int TX = %(TX)d, TY = %(TY)d;
   constexpr int LH = TX + 2 * K * SLEEVE, LW = TY + 2 * K * SLEEVE;
   constexpr int NTY = N2 / TY, NT = (N1 / TX) * NTY;
-  constexpr bool IDXARR = %(idxarr)s;
   static_assert(K <= 4, "levels 0 and 2 are given, 1 and 3 recomputed");
   static_assert((long long)N1 * N2 < (1LL << 31), "element offsets of a field are ints");
 %(fl)s
@@ -974,50 +972,28 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_unit(const char* __restrict
   double Pl[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) Pl[p] = HAS_PARAMS ? P[p] : 0.0;
-  constexpr int NL_IT = (LH * LW + %(ATH)d - 1) / %(ATH)d;
-  int rq[IDXARR ? NL_IT : 1], cq[IDXARR ? NL_IT : 1];
-  auto element = [&](int it, int& r, int& c) {
-    if constexpr (IDXARR) {
-      r = rq[it]; c = cq[it];
-    } else {
-      const int idx = threadIdx.x + it * %(ATH)d;
-      const int rr = idx / LW;
-      c = idx - rr * LW;
-      r = idx < LH * LW ? rr : LH;
-    }
-  };
-  if constexpr (IDXARR) {
-#pragma unroll
-    for (int it = 0; it < NL_IT; ++it) {
-      const int idx = threadIdx.x + it * %(ATH)d;
-      const int r = idx / LW;
-      rq[it] = idx < LH * LW ? r : LH;
-      cq[it] = idx - r * LW;
-    }
-  }
 %(lload)s
   %(pdecl)s
 #pragma unroll
   for (int mm = 0; mm < K; ++mm) {
-    const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} -> lam_m at state m
+    const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} (buffer mm & 1) -> lam_m
+    const int sb = mm & 1, db = sb ^ 1;
     const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
     // state m: recomputed (odd m) from the frame, read from the frame (even m
     // below an odd level), or read from HBM (even m = K-1)
     const bool recompute = (m & 1) != 0, framed = !recompute && m + 1 < K;
     if (recompute) {
+      if (mm > 0) __syncthreads();  // the level above has read the frame
       const char* __restrict__ xs = m == 1 ? st0 : ev;  // level m-1
-      constexpr int FR_MAX = LH;
       const int FR = TX + 2 * (m + 1) * SLEEVE;  // frame rows staged: [-2s, T + 2ms) of the tile
-      (void)FR_MAX;
 %(stage)s
     }
     __syncthreads();  // lam_{m+1} and the frame are in LDS
-    %(rdecl)s
 #pragma unroll
-    for (int it = 0; it < NL_IT; ++it) {
-      int r, c;
-      element(it, r, c);
-      if (r < HH && c < W) {
+    for (int it = 0; it < (HH * W + %(ATH)d - 1) / %(ATH)d; ++it) {
+      const int idx = threadIdx.x + it * %(ATH)d;
+      if (idx < HH * W) {
+        const int r = idx / W, c = idx - r * W;
         const int lo = r * LW + c, fo = (r + 2 * SLEEVE) * LW + (c + 2 * SLEEVE);
         %(xdecl)s
         if (recompute) {
@@ -1032,21 +1008,12 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_unit(const char* __restrict
         %(adecl)s
         adj_point<LW>(%(xargs)s, %(largs)s, lo, Pl, %(aargs)s);
         if (m > 0) {
-          %(rset)s
+          %(st_lds)s
         } else {
           const long long go = (long long)(x0 + r) * N2 + (y0 + c);
           %(st_glb)s
         }
         if (r < TX && c < TY) { %(pacc)s }
-      }
-    }
-    if (m > 0) {
-      __syncthreads();  // every read of lam_{m+1} (and of the frame) is done
-#pragma unroll
-      for (int it = 0; it < NL_IT; ++it) {
-        int r, c;
-        element(it, r, c);
-        if (r < HH && c < W) { const int lo = r * LW + c; %(st_lds)s }
       }
     }
   }

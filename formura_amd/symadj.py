@@ -200,6 +200,38 @@ class SymbolicAdjoint:
             out.append(self._grouped(terms) if terms else sp.Integer(0))
         return out
 
+    def _hoist(self, e: sp.Expr, defs: Dict[sp.Expr, sp.Symbol]) -> sp.Expr:
+        """``e`` with each maximal sum or product of parameters and literals
+        that is not a plain literal replaced by a symbol U_k (``defs``: its
+        definition), so that a point evaluates it as one value."""
+        uset = set(self.cvals) | set(self.psym)
+
+        def uniform(x):
+            return x.free_symbols <= uset
+
+        def const(x):
+            return x.free_symbols <= set(self.cvals)
+
+        def name(g):
+            if g not in defs:
+                defs[g] = sp.Symbol("U_%d" % len(defs))
+            return defs[g]
+
+        def walk(x):
+            if x.is_Atom:
+                return x
+            if uniform(x):
+                return x if const(x) else name(x)
+            if isinstance(x, (sp.Add, sp.Mul)):
+                us = [a for a in x.args if uniform(a)]
+                rest = [walk(a) for a in x.args if not uniform(a)]
+                if us and not all(const(a) for a in us):
+                    return x.func(name(x.func(*us)), *rest)
+                return x.func(*(us + rest))
+            return x.func(*[walk(a) for a in x.args])
+
+        return walk(e)
+
     def state_offsets(self) -> set:
         """Offsets (relative to q) at which the derived body reads the state:
         {(0, ..., 0)} for a step whose state-dependent partials and parameter
@@ -223,6 +255,16 @@ class SymbolicAdjoint:
         nf = len(self.fields)
         used = set()
         lines = []
+        # coefficients of the parameters (the same at every point) become one
+        # value each, evaluated in double and rounded once to the library's type
+        defs: Dict[sp.Expr, sp.Symbol] = {}
+        outs = [self._hoist(e, defs) for e in outs]
+        wide = self._printer(xload, lload, "double", used)
+        for g, u in defs.items():
+            if self.real == "double":
+                lines.append("const double %s = %s;" % (u.name, wide(g)))
+            else:
+                lines.append("const %s %s = (%s)%s;" % (self.real, u.name, self.real, wide(g)))
         names = list(acc_names) + list(pg_names)
         if self.real == "double":  # one pass: the two share their lam sums
             passes = ((outs, names, "double", "sa_"),)
@@ -294,9 +336,10 @@ class SymbolicAdjoint:
                         neg.append(c(-t))
                     else:
                         pos.append(c(t))
-                s = " + ".join(pos) if pos else lit(0)
-                for n in neg:
-                    s += " - " + n
+                if pos:
+                    s = " + ".join(pos) + "".join(" - " + n for n in neg)
+                else:
+                    s = "-" + " - ".join(neg)
                 return "(" + s + ")"
             if isinstance(e, sp.Mul):
                 cs = [t for t in e.args if const(t)]
