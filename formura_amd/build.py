@@ -97,6 +97,13 @@ PROGRAMS: Dict[str, tuple] = {
     "gs_64_tile_guard": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "div": "guard"}),
     # a level's work-items on a fixed column each; two rows per work-item
     "gs_64_tile_colmap": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "colmap": 1}),
+    # divisions without tests where a level's values passed the per-level range test
+    "gs_64_tile_level": ("grayscott2d.fmr", "gs_64.yaml",
+                         {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "colmap": 1, "div": "level"}),
+    "gs_64_tile_level_flat": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 16, "k": 3,
+                                                               "threads": 128, "div": "level"}),
+    "gs_2048_level": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "tile2d", "tx": 32, "ty": 32, "k": 4, "threads": 512,
+                                                         "colmap": 1, "div": "level"}),
     "gs_64_tile_k5_inplace_colmap": ("grayscott2d.fmr", "gs_64.yaml",
                                      {"kind": "tile2d", "tx": 16, "ty": 32, "k": 5, "inplace": 1, "colmap": 1}),
     "gs_64_tile_vpair": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "vpair": 1}),

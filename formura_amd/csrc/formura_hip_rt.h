@@ -111,6 +111,16 @@ __device__ __forceinline__ double fmr_divc1s(double a, double b, double y, unsig
   return fmr_divc1_fast(a, b, y);
 }
 
+// Per-level range test of the tile kernels: every value a level writes (and
+// every value of the loaded tile) is 0 or a finite x with 2^-848 <= |x| <= xmax.
+// Such values are integer multiples of 2^-900, so every dividend the generator
+// proved to be an integer combination of them (and of literals) is 0 or in
+// fmr_divc's range, and the next level divides with the fused sequences alone.
+__device__ __forceinline__ bool fmr_level_ok(double x, double xmax) {
+  const double ax = fabs(x);
+  return (ax >= 0x1p-848 && ax <= xmax) || ax == 0.0;
+}
+
 // Positivity checks of the mixed-precision form: every FP64 square-root
 // argument (e.g. gamma*p/rho of a wave speed) must be > 0; a non-positive or
 // NaN argument is counted (vector atomic, taken only on a violation).

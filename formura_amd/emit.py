@@ -186,6 +186,78 @@ def _divc_ok(b) -> bool:
     return math.isfinite(b) and 2.0 ** -100 <= abs(b) <= 2.0 ** 100
 
 
+# values whose every integer combination lands in fmr_divc's range or on zero
+LEVEL_LO = 2.0 ** -848
+
+
+def level_division_bound(node) -> "Optional[float]":
+    """For the per-level range test of the tile kernels (``div`` "level"): the
+    largest XMAX such that, if every value the node loads is 0 or a finite x
+    with 2^-848 <= |x| <= XMAX, every dividend of its FP64 divisions by
+    constants (fmr_divc / fmr_divc1) is 0 or in [2^-900, 2^900] -- or None if
+    a dividend is not an integer combination of loads and literals.
+
+    Why: a finite double with |x| >= 2^-848 is an integer multiple of 2^-900
+    (its ulp is at least 2^(-848-52)); so are literals of that size, their
+    integer multiples, and every rounded sum or difference of such multiples
+    (the exact result is a multiple, and rounding it to a coarser grid keeps
+    it one); a nonzero one has magnitude >= 2^-900.  A dividend sum_i c_i x_i
+    + sum_j k_j with integer c_i is bounded by sum |c_i| XMAX + sum |k_j| (to
+    rounding of the partial sums, covered by halving XMAX)."""
+    vals: Dict[int, object] = {}
+    for j, m in enumerate(node.micro):
+        ops = _operands(m)
+        if m.inst[0] == "Imm" or (ops and all(o in vals for o in ops)):
+            v = _fold(m, vals, m.typ)
+            if v is not None:
+                vals[j] = v
+    memo: Dict[int, object] = {}
+
+    def walk(j):
+        """(sum |c_i| over loads, sum |k_j| over literals) of micro j, or None."""
+        if j in memo:
+            return memo[j]
+        m = node.micro[j]
+        k = m.inst[0]
+        r = None
+        if j in vals:
+            v = vals[j]
+            if isinstance(v, float) and math.isfinite(v) and (v == 0.0 or abs(v) >= LEVEL_LO):
+                r = (0.0, abs(v))
+        elif k == "LoadCursorStatic":
+            r = (1.0, 0.0)
+        elif k == "Uniop" and m.inst[1] in ("-", "+"):
+            r = walk(m.inst[2])
+        elif k == "Binop" and m.inst[1] in ("+", "-"):
+            a, b = walk(m.inst[2]), walk(m.inst[3])
+            r = (a[0] + b[0], a[1] + b[1]) if a and b else None
+        elif k == "Binop" and m.inst[1] == "*":
+            for c, x in ((m.inst[2], m.inst[3]), (m.inst[3], m.inst[2])):
+                cv = vals.get(c)
+                if isinstance(cv, float) and cv == int(cv) and cv != 0.0 and abs(cv) <= 2.0 ** 20 and x not in vals:
+                    t = walk(x)
+                    r = (abs(cv) * t[0], abs(cv) * t[1]) if t else None
+                    break
+        memo[j] = r
+        return r
+
+    bound = None
+    for j, m in enumerate(node.micro):
+        if (m.inst[0] == "Binop" and m.inst[1] == "/" and m.inst[3] in vals and m.inst[2] not in vals
+                and CTYPE.get(m.typ, m.typ) == "double" and _divc_ok(vals[m.inst[3]])):
+            mant, _ = math.frexp(abs(float(vals[m.inst[3]])))
+            if mant == 0.5:
+                continue  # a power-of-two divisor is an exact product
+            w = walk(m.inst[2])
+            if w is None or w[1] >= 2.0 ** 899:
+                return None
+            if w[0] == 0.0:
+                continue
+            xmax = 2.0 ** math.floor(math.log2((2.0 ** 899 - w[1]) / w[0]))
+            bound = xmax if bound is None else min(bound, xmax)
+    return bound if bound is not None else 2.0 ** 896
+
+
 def _divfix_lines(ct, name, src, sg, fn, b, y, cast):
     """A division by a constant as the branch-free fused sequence, computed
     again with the guarded form where a lane of the wave holds a dividend
@@ -209,7 +281,8 @@ class Body:
     def __init__(self, node: MMNode, load: Callable, tmp_load: Callable, index: Callable,
                  store: Callable, prefix: str, fold: bool = True, const_kw="const ",
                  params: Dict[str, str] = None, real: str = None, recip: bool = False,
-                 prec: Dict[int, str] = None, divc: bool = True, spec_ok: str = None, divfix: bool = False):
+                 prec: Dict[int, str] = None, divc: bool = True, spec_ok: str = None, divfix: bool = False,
+                 divfast: bool = False):
         """``params``: literal names bound to run-time expressions (not folded);
         ``real``: if set ("float"), every floating micro value is computed in
         that type (the FP32 form; constants are folded in double and rounded,
@@ -233,7 +306,10 @@ class Body:
         ``divfix``: such divisions use the branch-free sequence and, where a
         lane of the wave holds a dividend outside its range, are computed
         again with the guarded form (a branch on a wave-uniform value, the
-        division alone redone)."""
+        division alone redone);
+        ``divfast``: such divisions use the branch-free sequence with no test
+        (the caller has established that every dividend is in range:
+        ``level_division_bound``)."""
         self.lines: List[str] = []
         sfx = (lambda fn: fn + "s") if spec_ok else (lambda fn: fn)
         sarg = (", " + spec_ok) if spec_ok else ""
@@ -315,12 +391,14 @@ class Body:
                     rhs = "%s*%s" % (src, show_double(1.0 / b))
                 else:
                     fn = sfx("fmr_divc1" if _divc_one_step(abs(b)) else "fmr_divc")
+                    if divfast:
+                        fn += "_fast"
                     self.spec_used |= bool(spec_ok)
                     if b > 0:
                         rhs = "%s(%s, %s, %s%s)" % (fn, src, show_double(b), show_double(1.0 / b), sarg)
                     else:
                         rhs = "-%s(%s, %s, %s%s)" % (fn, src, show_double(-b), show_double(-1.0 / b), sarg)
-                    if divfix:
+                    if divfix and not divfast:
                         self.lines += _divfix_lines(ct, name, src, "" if b > 0 else "-", fn, show_double(abs(b)),
                                                     show_double(1.0 / abs(b)), "")
                         continue
@@ -633,14 +711,16 @@ LDS_LOAD_LINE = re.compile(r"^const \w+ \w+ = lds\d+\[[^\]]*\];$")
 
 
 def tile_load(count: str, index_lines: List[str], pairs: List[tuple], ctype: str, tag: str = "ld",
-              threads: int = 256, indent: str = "  ", max_count: Optional[str] = None) -> str:
+              threads: int = 256, indent: str = "  ", max_count: Optional[str] = None, check: str = None) -> str:
     """C statements loading a tile of ``count`` elements (a constant expression)
     into LDS with ``threads`` work-items: every global load of the work-item is
     issued (unrolled, into registers) before the first LDS write, so the tile
     costs one memory latency, not one per loop trip.  ``index_lines`` compute
     ``g`` from ``idx``; ``pairs`` are (lds_element, global_element) in terms of
     ``idx`` and ``g``; all are of C type ``ctype``.  ``max_count`` (a constant
-    expression) bounds ``count`` when ``count`` is not itself constant."""
+    expression) bounds ``count`` when ``count`` is not itself constant.
+    ``check``: a statement applied to each loaded value (``%s`` is the value)
+    as it is written to LDS."""
     it = "%s_IT" % tag
     decl = ["%sconstexpr int %s = (%s + %d) / %d;" % (indent, it, max_count or count, threads - 1, threads)]
     decl.append("%s%s %s;" % (indent, ctype, ", ".join("%s_v%d[%s]" % (tag, i, it) for i in range(len(pairs)))))
@@ -659,6 +739,8 @@ def tile_load(count: str, index_lines: List[str], pairs: List[tuple], ctype: str
     out.append("%s  const int idx = threadIdx.x + it * %d;" % (indent, threads))
     out.append("%s  if (idx < (%s)) {" % (indent, count))
     out += ["%s    %s = %s_v%d[it];" % (indent, dst, tag, i) for i, (dst, src) in enumerate(pairs)]
+    if check:
+        out += ["%s    %s" % (indent, check % ("%s_v%d[it]" % (tag, i))) for i in range(len(pairs))]
     out.append("%s  }" % indent)
     out.append("%s}" % indent)
     return "\n".join(decl + issue + out)

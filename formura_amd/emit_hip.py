@@ -24,7 +24,7 @@ import json
 from typing import Dict, List
 
 from . import syntax as S
-from .emit import (LDS_LOAD_LINE, tile_load, mixed_precision, mixed_precision_graph, Body, EmitError, GraphPlan, c_elem_type, emit_header,
+from .emit import (level_division_bound, LDS_LOAD_LINE, tile_load, mixed_precision, mixed_precision_graph, Body, EmitError, GraphPlan, c_elem_type, emit_header,
                    lib_description, navi_fields, show_double, state_fields, CTYPE)
 from .manifest import MMNode, calc_range
 
@@ -596,8 +596,13 @@ class Tile2DSchedule:
         # (a wave-uniform branch per division).  "guard": a per-lane branch in
         # each division (fmr_divc1).  In every form the point's loads are
         # issued first.
-        if div not in ("spec", "fix", "guard"):
-            raise EmitError("div must be spec, fix or guard")
+        # "level": the values each level writes (and the loaded tile) are tested
+        # against fmr_level_ok; where no wave of the workgroup found one outside
+        # it, the next level divides with the fused sequences and no test (the
+        # generator proved every dividend an integer combination of such values,
+        # emit.level_division_bound), else with the guarded form.
+        if div not in ("spec", "fix", "guard", "level"):
+            raise EmitError("div must be spec, fix, guard or level")
         self.div = div
         # colmap: a level's work-items form P = NTH / W row groups of W columns
         # (W = the level's width); each owns one column and steps down rows
@@ -632,7 +637,7 @@ class Tile2DSchedule:
         outs = ", ".join("%s& o%d" % (ct, i) for i, (n, ct) in enumerate(fields))
         tbi = getattr(em, "tb_index", False)
 
-        def bodies(spec_ok):
+        def bodies(spec_ok, fast=False):
             loads, lines, used = [], [], False
             for k in plan.stores:
                 node = plan.mm[k]
@@ -653,7 +658,7 @@ class Tile2DSchedule:
 
                 b = Body(node, load, lambda *a: None, index if tbi else (lambda ax: None), store, "s%d_" % k,
                          prec=em.prec_of(node), recip=em.contract, spec_ok=spec_ok,
-                         divfix=self.div == "fix" and spec_ok is None)
+                         divfix=self.div == "fix" and spec_ok is None, divfast=fast)
                 used |= b.spec_used
                 # the node's loads (names unique to the node) before every node's arithmetic
                 loads += [l for l in b.lines if LDS_LOAD_LINE.search(l)]
@@ -662,7 +667,17 @@ class Tile2DSchedule:
                 lines.append("}")
             return loads + lines, used
 
-        body_lines, used = bodies("bad_" if self.div == "spec" else None)
+        level = self.div == "level"
+        xmax = None
+        if level:
+            if self.inplace or self.pair or self.vpair or tbi or em.contract or any(em.prec_of(plan.mm[k])
+                                                                               for k in plan.stores):
+                raise EmitError("div level needs the ping-pong tile2d (flat or column mapping) of the plain path")
+            bounds = [level_division_bound(plan.mm[k]) for k in plan.stores]
+            if any(b is None for b in bounds):
+                raise EmitError("div level: a dividend is not an integer combination of loads and literals")
+            xmax = min(bounds)
+        body_lines, used = bodies("bad_" if self.div == "spec" else None, fast=level)
         if self.div == "spec" and used:
             exact, _ = bodies(None)
             body_lines = (["unsigned long long bad_ = 0;  // lanes with a dividend out of range"] + body_lines
@@ -671,15 +686,26 @@ class Tile2DSchedule:
         ipar = ", const int gx, const int gy, const int ib0, const int ib1" if tbi else ""
         out.append("template <int RY>\n__device__ __forceinline__ void pt_step(%s, const int o%s, %s) {\n%s\n}\n"
                    % (params, ipar, outs, "\n".join("  " + l for l in body_lines)))
+        if level:
+            guarded, _ = bodies(None)
+            out.append("template <int RY>\n__device__ __forceinline__ void pt_step_g(%s, const int o%s, %s) {\n%s\n}\n"
+                       % (params, ipar, outs, "\n".join("  " + l for l in guarded)))
         wrap1 = em.wrap_fn(0, 2 * self.k * s + 1)
         wrap2 = em.wrap_fn(1, 2 * self.k * s + 1)
         nb = 1 if self.inplace else 2
         lds = "\n".join("  __shared__ %s B%d[%d][RX * RY];" % (ct, i, nb) for i, (n, ct) in enumerate(fields))
+        lcheck = ("lb0 |= !fmr_level_ok(%%s, %s);" % show_double(xmax)) if level else None
         load_tile = tile_load("RX * RY", ["const int r = idx / RY, c = idx - r * RY;",
                                           "const long long g = (long long)%s(x0 - H + r) * N2 + %s(y0 - H + c);"
                                           % (wrap1, wrap2)],
                               [("B%d[0][idx]" % i, "FLD%d(st_in)[g]" % i) for i in range(len(fields))],
-                              fields[0][1], "ld", threads=self.threads)
+                              fields[0][1], "ld", threads=self.threads, check=lcheck)
+        if level:
+            load_tile = ("  bool lb0 = false;  // a loaded value outside fmr_level_ok\n" + load_tile +
+                         "\n  { const unsigned long long m = __builtin_amdgcn_ballot_w64(lb0);\n"
+                         "    if ((threadIdx.x & 63) == 0) wbad[0][threadIdx.x >> 6] = m != 0ull; }")
+            lds += "\n  __shared__ int wbad[K][%d];  // wave w found a value of level j outside fmr_level_ok" \
+                % (self.threads // 64)
         srcs = ", ".join("B%d[src]" % i for i in range(len(fields)))
         if tbi:
             srcs += ", o, x0 + r - H, y0 + c - H, ib0, ib1"
@@ -723,9 +749,18 @@ class Tile2DSchedule:
                 st_lds=st_lds + " " + " ".join("B%d[dst][o + RY] = p%d;" % (i, i) for i in range(len(fields))),
                 st_glb=st_glb + " " + " ".join("FLD%d(st_out)[g + N2] = p%d;" % (i, i) for i in range(len(fields)))))
         else:
-            levels = LEVELS_PINGPONG % dict(ibdecl=ibdecl, loop=(TILE2D_COLS_LOOP if self.colmap else TILE2D_FLAT_LOOP)
-                                            % dict(NTH=self.threads, decl_o=decl_o, srcs=srcs, refs_o=refs_o,
-                                                   st_lds=st_lds, st_glb=st_glb))
+            loop = TILE2D_COLS_LOOP if self.colmap else TILE2D_FLAT_LOOP
+            tmpl = LEVELS_PINGPONG
+            if level:
+                # the level's loop twice, the branch on the workgroup-uniform flag outside it
+                loop = ("    if (safe) {\n" + loop + "\n    } else {\n"
+                        + loop.replace("pt_step<RY>(%(srcs)s, %(refs_o)s);", "pt_step_g<RY>(%(srcs)s, %(refs_o)s);")
+                        + "\n    }")
+                st_lds = st_lds + " lb |= " + " | ".join("!fmr_level_ok(o%d, %s)" % (i, show_double(xmax))
+                                                        for i in range(len(fields))) + ";"
+                tmpl = LEVELS_PINGPONG_LEVEL.replace("NW_", str(self.threads // 64))
+            levels = tmpl % dict(ibdecl=ibdecl, loop=loop % dict(NTH=self.threads, decl_o=decl_o, srcs=srcs,
+                                                                 refs_o=refs_o, st_lds=st_lds, st_glb=st_glb))
         head = TILE2D_HEAD % dict(NTH=self.threads, TX=TX, TY=TY, lds=lds,
                                   offp=", const int offT0, const int offT1" if tbi else "")
         out.append(head + TILE2D_ONE % dict(load_tile=load_tile, levels=levels))
@@ -854,6 +889,26 @@ LEVELS_PINGPONG = """#pragma unroll
     const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;%(ibdecl)s
 %(loop)s
     if (j < K) __syncthreads();
+  }"""
+
+# the same with the per-level range test (div "level"): level j divides without
+# tests when no wave found a value of level j-1 (or of the loaded tile) outside
+# fmr_level_ok
+LEVELS_PINGPONG_LEVEL = """#pragma unroll
+  for (int j = 1; j <= K; ++j) {
+    const int src = (j - 1) & 1, dst = j & 1;
+    const int lo = 2 * j * SLEEVE, W = RY - lo, HH = RX - lo;%(ibdecl)s
+    int nbad = 0;
+#pragma unroll
+    for (int w = 0; w < NW_; ++w) nbad |= wbad[j - 1][w];
+    const bool safe = __builtin_amdgcn_readfirstlane(nbad) == 0;  // the same in every lane
+    bool lb = false;
+%(loop)s
+    if (j < K) {
+      const unsigned long long m = __builtin_amdgcn_ballot_w64(lb);
+      if ((threadIdx.x & 63) == 0) wbad[j][threadIdx.x >> 6] = m != 0ull;
+      __syncthreads();
+    }
   }"""
 
 # K levels in one LDS buffer: a level's values go to registers and overwrite

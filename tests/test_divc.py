@@ -17,6 +17,9 @@ condition rests on the proof in the header, not on this test.
 """
 import os
 import subprocess
+from fractions import Fraction
+
+import numpy as np
 
 import pytest
 
@@ -70,3 +73,61 @@ def test_one_step_condition():
     from formura_amd.emit import _divc_one_step
     assert _divc_one_step(1e-4) and _divc_one_step(0.01) and _divc_one_step(3.0)
     assert not _divc_one_step(1e-5) and not _divc_one_step(9.999999999999999e-5)
+
+
+# --- the per-level range test of the tile kernels (div "level") -----------------
+
+def _step_nodes(name):
+    import os
+    from formura_amd import build as B
+    from formura_amd.emit_hip import generate
+    fmr, yml, _ = B.PROGRAMS[name]
+    _, _, mm = generate(os.path.join(B.PROG_DIR, fmr), os.path.join(B.PROG_DIR, yml), name, "flat")
+    return mm.step
+
+
+def test_level_division_bound_grayscott_and_planted_dividends():
+    """Gray-Scott's dividends (a + b) - 2c: XMAX = 2^897 (4 XMAX <= 2^899).  A
+    dividend with a non-integer factor, or a product of loads, has no bound."""
+    import copy
+    from formura_amd.emit import level_division_bound
+    nodes = _step_nodes("gs_64")
+    assert [level_division_bound(n) for _, n in sorted(nodes.items())] == [2.0 ** 897] * 2
+    node = copy.deepcopy(sorted(nodes.items())[0][1])
+    # the literal 2.0 of a 2*c term of a dividend becomes 2.5: not an integer combination
+    twos = [mm.inst[2] for mm in node.micro if mm.inst[0] == "Binop" and mm.inst[1] == "*"
+            and node.micro[mm.inst[2]].inst[0] == "Imm" and float(Fraction(node.micro[mm.inst[2]].inst[1])) == 2.0
+            and node.micro[mm.inst[3]].inst[0] == "LoadCursorStatic"]
+    assert twos
+    m = node.micro[twos[0]]
+    m.inst = ("Imm", Fraction(5, 2)) + tuple(m.inst[2:])
+    assert level_division_bound(node) is None
+    node = copy.deepcopy(sorted(nodes.items())[0][1])
+    # a dividend term c*2 -> c*c (a product of loads)
+    muls = [j for j, mm in enumerate(node.micro) if mm.inst[0] == "Binop" and mm.inst[1] == "*"
+            and node.micro[mm.inst[2]].inst[0] == "Imm" and node.micro[mm.inst[3]].inst[0] == "LoadCursorStatic"]
+    j = muls[0]
+    a, b = node.micro[j].inst[2], node.micro[j].inst[3]
+    node.micro[j].inst = ("Binop", "*", b, b)
+    assert level_division_bound(node) is None
+
+
+def test_level_values_give_dividends_in_range():
+    """Values that pass fmr_level_ok (0, or 2^-848 <= |x| <= 2^897) give second
+    differences (a + b) - 2c in double that are 0 or within [2^-900, 2^899]:
+    exact rational check on values near both ends of the range and on mixtures."""
+    rng = np.random.default_rng(5)
+    lo, hi = 2.0 ** -848, 2.0 ** 897
+    pools = [lo * (1 + rng.random(4000)), lo * 2.0 ** rng.integers(0, 60, 4000) * (1 + rng.random(4000)),
+             hi * rng.random(4000), np.zeros(100), 1.0 + rng.random(1000)]
+    vals = np.concatenate(pools)
+    vals *= np.where(rng.random(vals.size) < 0.5, -1.0, 1.0)
+    vals = vals[(np.abs(vals) >= lo) & (np.abs(vals) <= hi) | (vals == 0)]
+    n = 200000
+    a, b, c = (vals[rng.integers(0, vals.size, n)] for _ in range(3))
+    d = (a + b) - 2.0 * c
+    nz = d[d != 0]
+    assert np.all(np.abs(nz) >= 2.0 ** -900) and np.all(np.abs(nz) <= 2.0 ** 899) and np.all(np.isfinite(d))
+    # every nonzero one is a multiple of 2^-900 (checked exactly on a sample)
+    for x in nz[:2000]:
+        assert (Fraction(float(x)) / Fraction(1, 2 ** 900)).denominator == 1

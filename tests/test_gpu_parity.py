@@ -110,6 +110,8 @@ def _gs_extreme_seed(lib):
     ("gs_64_tile_guard", "gs_64", 10, True),    # a per-lane branch in each division
     ("gs_64_tile_colmap", "gs_64", 10, True),
     ("gs_64_tile_vpair", "gs_64", 10, True),
+    ("gs_64_tile_level", "gs_64", 10, True),    # per-level range test, workgroups with such values guarded
+    ("gs_64_tile_level_flat", "gs_64", 9, True),
     ("gs_64_flat", "gs_64", 7, False),          # the guarded divisions of the flat kernels
 ])
 def test_grayscott_out_of_range_dividends(name, oracle, steps, resident):
@@ -152,6 +154,9 @@ def test_resident_multi_step_launches(name, steps):
     ("gs_64_tile_colmap", "gs_64", 43, True),         # a fixed column per work-item
     ("gs_64_tile_k5_inplace_colmap", "gs_64", 45, True),
     ("gs_64_tile_vpair", "gs_64", 39, True),          # two rows of one column per work-item
+    ("gs_64_tile_level", "gs_64", 45, True),          # divisions untested where a level's values passed
+    ("gs_64_tile_level_flat", "gs_64", 41, True),
+    ("gs_2048_level", "gs_2048", 43, True),
 ])
 def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     seed = (lambda lib: inputs.heat_field(13, lib.grid)) if name.startswith("heat") else _gs_seed
