@@ -87,6 +87,28 @@ __host__ __device__ __forceinline__ double fmr_divc1(double a, double b, double 
   return fmr_divc1_fast(a, b, y);
 }
 
+// The division again for a wave that holds a dividend outside the fused
+// sequence's range.  A dividend with lo <= |a| < 2^-900 is scaled by 2^600
+// (exact: the result is normal, below 2^-300), divided with the fused sequence
+// (its range holds it), and the quotient scaled back by 2^-600 (exact: the
+// generator passes lo = 2^ceil(log2 b) * 2^-1021, so |a/b| >= 2^-1021 and the
+// quotient stays normal, where scaling by a power of two commutes with
+// rounding); every other out-of-range dividend (smaller, huge, Inf, NaN) takes
+// the division.  Lanes already in range get the fused sequence unscaled.
+__host__ __device__ __forceinline__ double fmr_divc_redo(double a, double b, double y, double lo) {
+  const double aa = fabs(a);
+  const bool tiny = aa < 0x1p-900 && aa >= lo;
+  if (__builtin_expect(!(tiny || fmr_divc_in_range(a)), 0)) return a / b;
+  return fmr_divc_fast(a * (tiny ? 0x1p600 : 1.0), b, y) * (tiny ? 0x1p-600 : 1.0);
+}
+
+__host__ __device__ __forceinline__ double fmr_divc1_redo(double a, double b, double y, double lo) {
+  const double aa = fabs(a);
+  const bool tiny = aa < 0x1p-900 && aa >= lo;
+  if (__builtin_expect(!(tiny || fmr_divc_in_range(a)), 0)) return a / b;
+  return fmr_divc1_fast(a * (tiny ? 0x1p600 : 1.0), b, y) * (tiny ? 0x1p-600 : 1.0);
+}
+
 // Speculative forms for point functions: the fused sequences without the
 // branch; the lanes whose dividend is not fmr_divc_in_range are or-ed into the
 // wave mask `bad` (one ballot per comparison, each a single compare into a lane

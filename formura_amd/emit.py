@@ -258,13 +258,23 @@ def level_division_bound(node) -> "Optional[float]":
     return bound if bound is not None else 2.0 ** 896
 
 
+def redo_lower_bound(b: float) -> float:
+    """Smallest |a| the scaled redo (fmr_divc_redo) divides with the fused
+    sequence for the divisor magnitude b: 2^ceil(log2 b) * 2^-1021 >= b 2^-1021,
+    so that |a/b| >= 2^-1021 (a normal quotient)."""
+    m, e = math.frexp(abs(b))
+    return math.ldexp(1.0, (e - 1 if m == 0.5 else e) - 1021)
+
+
 def _divfix_lines(ct, name, src, sg, fn, b, y, cast):
     """A division by a constant as the branch-free fused sequence, computed
-    again with the guarded form where a lane of the wave holds a dividend
-    outside its range (a wave-uniform branch)."""
+    again where a lane of the wave holds a dividend outside its range (a
+    wave-uniform branch): tiny dividends through the scaled fused sequence,
+    the rest with the division (fmr_divc_redo)."""
+    lo = show_double(redo_lower_bound(float(b)))
     return ["%s %s = %s(%s%s_fast(%s, %s, %s));" % (ct, name, cast, sg, fn, src, b, y),
-            "if (__builtin_expect(fmr_divc_out_of_range(%s) != 0ull, 0)) %s = %s(%s%s(%s, %s, %s));"
-            % (src, name, cast, sg, fn, src, b, y)]
+            "if (__builtin_expect(fmr_divc_out_of_range(%s) != 0ull, 0)) %s = %s(%s%s_redo(%s, %s, %s, %s));"
+            % (src, name, cast, sg, fn, src, b, y, lo)]
 
 
 def _const_literal(v, typ):

@@ -60,6 +60,26 @@ int main(int argc, char** argv) {
         if (bits(q3) != bits(q2)) { if (bad < 10) printf("MISMATCH (one step) b=%a a=%a got %a want %a\n", b, a, q3, q2); ++bad; }
       }
     }
+    // the redo of a wave with out-of-range dividends: tiny and subnormal ones
+    // (scaled by 2^600 where |a| >= lo, else the division), huge ones
+    {
+      int e;
+      const double m = frexp(b, &e);
+      const double lo = ldexp(1.0, (m == 0.5 ? e - 1 : e) - 1021);
+      const long nr = bi < NFIX ? n / 4 : n / 64;
+      for (long i = 0; i < nr; ++i) {
+        uint64_t r = nx();
+        double a;
+        if (i % 4 == 3) a = dbl(r >> 12);  // subnormals and tiny normals, random bits
+        else a = ldexp(1.0 + (double)(nx() >> 12) * 0x1p-52, (int)(r % 200) - 1100);  // 2^-1100 .. 2^-901
+        if (i % 5 == 4) a = ldexp(1.0 + (double)(nx() >> 12) * 0x1p-52, 900 + (int)(r % 120));  // huge
+        if (r & (1ull << 62)) a = -a;
+        const double want = a / b;
+        ++tot;
+        if (bits(fmr_divc_redo(a, b, y, lo)) != bits(want)) { if (bad < 10) printf("MISMATCH (redo) b=%a a=%a got %a want %a\n", b, a, fmr_divc_redo(a, b, y, lo), want); ++bad; }
+        if (one && bits(fmr_divc1_redo(a, b, y, lo)) != bits(want)) { if (bad < 10) printf("MISMATCH (redo, one step) b=%a a=%a\n", b, a); ++bad; }
+      }
+    }
     double sp[] = {0.0, -0.0, 1e-310, -1e-310, 1e308, -1e308, INFINITY, -INFINITY, NAN, 0x1p-900, 0x1p900, 0x1p-901, 0x1.fffffffffffffp900, 5e-324};
     // the range test on the specials: in range exactly the signed zeros and 2^-900, 2^900
     for (double a : sp) {
@@ -71,6 +91,10 @@ int main(int argc, char** argv) {
       ++tot;
       if (bits(q1) != bits(q2) && !(q1 != q1 && q2 != q2)) { printf("SPECIAL MISMATCH b=%a a=%a got %a want %a\n", b, a, q1, q2); ++bad; }
       if (bits(q3) != bits(q2) && !(q3 != q3 && q2 != q2)) { printf("SPECIAL MISMATCH (one step) b=%a a=%a got %a want %a\n", b, a, q3, q2); ++bad; }
+      int e;
+      const double mb = frexp(b, &e);
+      const double q5 = fmr_divc_redo(a, b, y, ldexp(1.0, (mb == 0.5 ? e - 1 : e) - 1021));
+      if (bits(q5) != bits(q2) && !(q5 != q5 && q2 != q2)) { printf("SPECIAL MISMATCH (redo) b=%a a=%a got %a want %a\n", b, a, q5, q2); ++bad; }
     }
   }
   printf("divisors %d, one-step divisors %ld\n", NFIX + NRND, one_step);

@@ -55,8 +55,8 @@ def test_emitted_divisions():
                          os.path.join(build.PROG_DIR, "gs_64.yaml"), "gs_64")
     # d_xx and d_yy of u and of v: the fused sequence, and the guarded form
     # for the waves that hold a dividend outside its range
-    assert src.count("fmr_divc1_fast(") == 4 and src.count("fmr_divc1(") == 4
-    assert src.count(", 0.0001, 10000.0)") == 8 and src.count("fmr_divc_out_of_range(") == 4
+    assert src.count("fmr_divc1_fast(") == 4 and src.count("fmr_divc1_redo(") == 4
+    assert src.count(", 0.0001, 10000.0") == 8 and src.count("fmr_divc_out_of_range(") == 4
     _, fsrc, _ = generate(os.path.join(build.PROG_DIR, "grayscott2d.fmr"),
                           os.path.join(build.PROG_DIR, "gs_64.yaml"), "gs_64", "flat")
     assert fsrc.count("fmr_divc1(") == 4 and "fmr_divc1s(" not in fsrc
