@@ -133,6 +133,22 @@ __device__ __forceinline__ double fmr_divc1s(double a, double b, double y, unsig
   return fmr_divc1_fast(a, b, y);
 }
 
+// Division by a computed value (not a literal) in the opt-in contract form:
+// the hardware reciprocal approximation refined by two Newton steps, the
+// quotient, and one correction of it.  For normal operands and a normal
+// quotient the result is within a few units in the last place of IEEE
+// division, without the device's scaling and special-case steps; zero,
+// infinite, subnormal and NaN operands are not handled as IEEE division
+// handles them.  Only the contract form, checked against the exact path to a
+// stated tolerance, uses it.
+__device__ __forceinline__ double fmr_div_contract(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+
 // Per-level range test of the tile kernels: every value a level writes (and
 // every value of the loaded tile) is 0 or a finite x with 2^-848 <= |x| <= xmax.
 // Such values are integer multiples of 2^-900, so every dividend the generator

@@ -292,7 +292,7 @@ class Body:
                  store: Callable, prefix: str, fold: bool = True, const_kw="const ",
                  params: Dict[str, str] = None, real: str = None, recip: bool = False,
                  prec: Dict[int, str] = None, divc: bool = True, spec_ok: str = None, divfix: bool = False,
-                 divfast: bool = False):
+                 divfast: bool = False, fastdiv: bool = False):
         """``params``: literal names bound to run-time expressions (not folded);
         ``real``: if set ("float"), every floating micro value is computed in
         that type (the FP32 form; constants are folded in double and rounded,
@@ -319,7 +319,10 @@ class Body:
         division alone redone);
         ``divfast``: such divisions use the branch-free sequence with no test
         (the caller has established that every dividend is in range:
-        ``level_division_bound``)."""
+        ``level_division_bound``);
+        ``fastdiv``: an FP64 division by a computed value is
+        ``fmr_div_contract`` (reciprocal approximation and Newton steps; the
+        opt-in contract form only, not IEEE)."""
         self.lines: List[str] = []
         sfx = (lambda fn: fn + "s") if spec_ok else (lambda fn: fn)
         sarg = (", " + spec_ok) if spec_ok else ""
@@ -459,6 +462,11 @@ class Body:
                     rhs = "%s%s" % (op, a(m.inst[2]))
             elif k == "Binop":
                 op = m.inst[1]
+                if (fastdiv and op == "/" and ct == "double" and m.inst[3] not in vals
+                        and ctypes.get(m.inst[2]) == "double" and ctypes.get(m.inst[3]) == "double"):
+                    self.lines.append("%s%s %s = fmr_div_contract(%s, %s);" % (const_kw, ct, name, a(m.inst[2]),
+                                                                             a(m.inst[3])))
+                    continue
                 if op == "**":
                     pt = ct if prec else (real or "double")
                     rhs = "pow((%s)%s,(%s)%s)" % (pt, a(m.inst[2]), pt, a(m.inst[3]))

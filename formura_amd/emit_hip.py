@@ -205,7 +205,7 @@ class LibEmitter:
             load, tmp_load = loaders([0] * dim)
             precs = [self.prec_of(plan.mm[k]) or {} for k in grp] if self.mixed else None
             node, roots, mprec = _merge_nodes([plan.mm[k] for k in grp], precs)
-            body = Body(node, load, tmp_load, lambda ax: "0", lambda n, v: "", "a", prec=mprec, recip=self.contract)
+            body = Body(node, load, tmp_load, lambda ax: "0", lambda n, v: "", "a", prec=mprec, recip=self.contract, fastdiv=self.contract)
             body.lines += ["t_%d[t] = (%s)a%d;" % (k, plan.temp_ctype(k), roots[i]) for i, k in enumerate(grp)]
             deps = temp_group_inputs(plan, grp)
             outs_ = "".join(", %s* __restrict__ t_%d" % (plan.temp_ctype(k), k) for k in grp)
@@ -229,7 +229,7 @@ class LibEmitter:
             def index(ax):
                 return "(%s+off_%d)" % (ivs[ax], ax + 1)
 
-            body = Body(node, load, tmp_load, index, store, "s%d_" % k, prec=self.prec_of(node), recip=self.contract)
+            body = Body(node, load, tmp_load, index, store, "s%d_" % k, prec=self.prec_of(node), recip=self.contract, fastdiv=self.contract)
             lines.append("{")
             lines += ["  " + l for l in body.lines]
             lines.append("}")
@@ -657,7 +657,7 @@ class Tile2DSchedule:
                     return "(%s + ib%d)" % ("gx" if ax == 0 else "gy", ax)
 
                 b = Body(node, load, lambda *a: None, index if tbi else (lambda ax: None), store, "s%d_" % k,
-                         prec=em.prec_of(node), recip=em.contract, spec_ok=spec_ok,
+                         prec=em.prec_of(node), recip=em.contract, fastdiv=em.contract, spec_ok=spec_ok,
                          divfix=self.div == "fix" and spec_ok is None, divfast=fast)
                 used |= b.spec_used
                 # the node's loads (names unique to the node) before every node's arithmetic
@@ -1119,7 +1119,7 @@ class Wave2DSchedule:
                     return "n%d = %s;" % (em.field_index[fname], v)
 
                 b = Body(node, load, lambda *a: None, lambda ax: None, store, "%s%d_" % (prefix, k),
-                         prec=em.prec_of(node), recip=em.contract)
+                         prec=em.prec_of(node), recip=em.contract, fastdiv=em.contract)
                 lines.append("{")
                 lines += ["  " + l for l in b.lines]
                 lines.append("}")
@@ -1383,7 +1383,7 @@ class Staged3DSchedule:
                 return "S%d[sl_%d_%s][so + (%d)*%d + (%d)]" % (fi, gi, str(c[0]).replace("-", "m"), c[1], st["SRZ"], c[2])
 
             b = Body(node, load, lambda *a: None, lambda ax: None, lambda n, v: "", "g%d_" % gi, prec=mprec,
-                     recip=em.contract)
+                     recip=em.contract, fastdiv=em.contract)
             kw = "if" if gi == 0 else "} else if"
             # planes of this group computed before iteration x0 - (DG - 1) are never read
             lines = ["%s (gidx < %d) {  // group %d: temporaries %s, read at x offsets %d..%d"
@@ -1425,7 +1425,7 @@ class Staged3DSchedule:
             def store(fname, v):
                 return "FLD%d(st_out)[gout] = %s;" % (em.field_index[fname], v)
 
-            b = Body(node, sload, tload, lambda ax: None, store, "u%d_" % k, prec=em.prec_of(node), recip=em.contract)
+            b = Body(node, sload, tload, lambda ax: None, store, "u%d_" % k, prec=em.prec_of(node), recip=em.contract, fastdiv=em.contract)
             sbody += ["{"] + ["  " + l for l in b.lines] + ["}"]
         slot_lines = []
         for kind, o in sorted(xslots):
@@ -1554,7 +1554,7 @@ class Stream3DSchedule:
                 return "(g%d + ib%d)" % (ax, ax)
 
             b = Body(node, load, lambda *a: None, index if tbi else (lambda ax: None), store, "s%d_" % k,
-                     prec=em.prec_of(node), recip=em.contract)
+                     prec=em.prec_of(node), recip=em.contract, fastdiv=em.contract)
             body_lines.append("{")
             body_lines += ["  " + l for l in b.lines]
             body_lines.append("}")

@@ -1,11 +1,13 @@
 """The opt-in contract form (spec key ``contract``): FMA contraction across the
-instruction list and division by a constant as the product with its
-reciprocal.  It is not bit-exact, so it is checked against the plain FP64
-path -- which the parity tests hold bit-exact with the oracle -- to the north
-star's FP64 tolerance, 1e-12 relative max-norm per field, at the stated steps:
+instruction list, division by a constant as the product with its reciprocal,
+and division by a computed value through the hardware reciprocal refined by
+Newton steps (``fmr_div_contract``).  It is not bit-exact, so it is checked
+against the plain FP64 path -- which the parity tests hold bit-exact with the
+oracle -- to the FP64 tolerance of 1e-12 relative max-norm per field, at the
+stated steps:
 
 * Euler HLLC 256^3, 200 steps (every division of the flux is by a state
-  value, so only contraction changes the arithmetic);
+  value: contraction and the refined reciprocal change the arithmetic);
 * Gray-Scott 64^2, 300 steps.
 
 Gray-Scott 2048^2 over its full 10^4 steps does not meet 1e-12 in this form
