@@ -36,8 +36,11 @@ void Formura_AD_Adjoint(const void *state_prev, const void *lam_next, void *lam_
  * with one-point-per-work-item kernels). */
 int Formura_AD_Levels(void);
 /* nsteps (1..Formura_AD_Levels()) adjoint steps in one call from the state at
- * their start: the inner states are recomputed into scratch (nsteps-1 state
- * buffers); lam_prev = dL/d(state0); pgrad += dL/dparams of the nsteps steps. */
+ * their start; scratch is work space of nsteps-1 state buffers for the inner
+ * states the call recomputes (a library whose adjoint reads the state at the
+ * point only writes the state two steps in and recomputes the others inside
+ * its adjoint launch); lam_prev = dL/d(state0); pgrad += dL/dparams of the
+ * nsteps steps. */
 void Formura_AD_Adjoint_Steps(const void *state0, void *scratch, const void *lam_next, void *lam_prev, int nsteps,
                               const double *params_dev, double *pgrad, void *stream);
 /* The two halves of Formura_AD_Adjoint_Steps, for a caller that runs the
@@ -51,9 +54,10 @@ void Formura_AD_Adjoint_Levels(const void *state0, const void *scratch, const vo
 /* Executes a reversal schedule: nops operations of 4 ints (kind, a, b, k):
  *   kind 0: bufs[b] = k forward steps from bufs[a]                (stream_f)
  *   kind 1: the adjoint of the k (<= Formura_AD_Levels()) steps that start
- *           at bufs[a]: inner states on stream_f into one half of scratch
- *           (2*(levels-1) state buffers, halves used in turn), adjoint steps
- *           on stream_a from lam to lam2 (then the two swap)   (stream_a)
+ *           at bufs[a]: inner states (or the one state the fused adjoint
+ *           needs) on stream_f into one half of scratch (2*(levels-1) state
+ *           buffers, halves used in turn), adjoint steps on stream_a from
+ *           lam to lam2 (then the two swap)                      (stream_a)
  * Events order the two streams on every shared buffer, so the forward work of
  * later operations overlaps the adjoint ones (stream_f == stream_a: one
  * stream, no events).  lam holds dL/d(output) on entry; returns 0 if the
