@@ -57,6 +57,14 @@ PROGRAMS: Dict[str, tuple] = {
     "heat3d_32_stream": ("heat3d.fmr", "heat3d_32.yaml", {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 2}),
     "heat3d_64_tb_stream": ("heat3d.fmr", "heat3d_64_tb.yaml", {"kind": "stream3d", "ty": 4, "tz": 16, "cx": 16, "k": 3}),
     "heat3d_32_stream512": ("heat3d.fmr", "heat3d_32.yaml", {"kind": "stream3d", "ty": 16, "tz": 32, "cx": 8, "k": 2, "threads": 512}),
+    # skewed levels: one barrier per plane (rings of 2s+2 planes)
+    "heat3d_32_skew": ("heat3d.fmr", "heat3d_32.yaml", {"kind": "stream3d", "ty": 8, "tz": 16, "cx": 8, "k": 2, "skew": 1}),
+    "heat3d_64_tb_skew": ("heat3d.fmr", "heat3d_64_tb.yaml", {"kind": "stream3d", "ty": 4, "tz": 16, "cx": 16, "k": 3,
+                                                             "skew": 1}),
+    "heat3d_24x40x56_skew": ("heat3d.fmr", "heat3d_24x40x56.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8,
+                                                                   "k": 2, "skew": 1}),
+    "heat3d_kahan_16_skew": ("heat3d_kahan.fmr", "heat3d_kahan_16.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8,
+                                                                         "k": 2, "skew": 1}),
     "gs_64_flat": ("grayscott2d.fmr", "gs_64.yaml", "flat"),
     # grids no power of two divides: 72 x 120 and 24 x 40 x 56
     "gs_72x120": ("grayscott2d.fmr", "gs_72x120.yaml", "auto"),
@@ -69,7 +77,8 @@ PROGRAMS: Dict[str, tuple] = {
     # FP32 state with Kahan-compensated accumulation written in the language (fields q, c)
     "heat3d_kahan_16": ("heat3d_kahan.fmr", "heat3d_kahan_16.yaml", "flat"),
     "heat3d_kahan_16_stream": ("heat3d_kahan.fmr", "heat3d_kahan_16.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8, "k": 2}),
-    "heat3d_kahan_512": ("heat3d_kahan.fmr", "heat3d_kahan_512.yaml", {"kind": "stream3d", "ty": 16, "tz": 64, "cx": 64, "k": 2, "threads": 512}),
+    "heat3d_kahan_512": ("heat3d_kahan.fmr", "heat3d_kahan_512.yaml", {"kind": "stream3d", "ty": 32, "tz": 64, "cx": 256, "k": 2,
+                                                                        "skew": 1, "threads": 1024}),
     "diffusion2d_64_wave": ("diffusion2d.fmr", "diffusion2d_64.yaml", {"kind": "wave2d", "rows": 8, "k": 3}),
     "heat3d_24x40x56_tb_stream": ("heat3d.fmr", "heat3d_24x40x56_tb.yaml", {"kind": "stream3d", "ty": 8, "tz": 8, "cx": 8, "k": 2}),
     "euler3d_16_inline": ("euler3d.fmr", "euler3d_16.yaml", {"kind": "flat", "inline": 1}),

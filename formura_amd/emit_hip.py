@@ -1524,14 +1524,23 @@ class Stream3DSchedule:
 
     name = "stream3d"
 
-    def __init__(self, ty, tz, cx, k, threads=256):
+    def __init__(self, ty, tz, cx, k, threads=256, skew=False):
         self.ty, self.tz, self.cx, self.k = ty, tz, cx, k
         # work-items per workgroup
         self.threads = int(threads)
+        # skew: level j computes plane t - (j - 1) in the trip where level 1
+        # computes plane t, so that every level reads only planes finished in
+        # earlier trips: one barrier per trip instead of K + 1, for rings of
+        # 2s + 2 planes (one more)
+        self.skew = bool(skew)
 
     def kernels(self, em, plan):
         s = plan.sleeve
         R = 2 * s + 1
+        if self.skew:
+            if getattr(em, "tb_index", False):
+                raise EmitError("stream3d: skew does not take index reads")
+            R = 2 * s + 2
         fields = em.fields
         params = ", ".join("const %s* __restrict__ lds%d" % (ct, i) for i, (n, ct) in enumerate(fields))
         outs = ", ".join("%s& o%d" % (ct, i) for i, (n, ct) in enumerate(fields))
@@ -1574,6 +1583,13 @@ class Stream3DSchedule:
         refs_o = ", ".join("o%d" % i for i in range(nf))
         st_lds = " ".join("B%d[j][slot * PLANE + o] = o%d;" % (i, i) for i in range(nf))
         st_glb = " ".join("FLD%d(st_out)[g] = o%d;" % (i, i) for i in range(nf))
+        if self.skew:
+            st_lds_k = " ".join("B%d[j][ws * PLANE + o] = o%d;" % (i, i) for i in range(nf))
+            out.append(STREAM3D_SKEW % dict(TY=self.ty, TZ=self.tz, CX=self.cx, R=R, NTH=self.threads, lds=lds,
+                                            w0=wrap[0], w1=wrap[1], w2=wrap[2], pf_decl=pf_decl, pf_load=pf_load,
+                                            pf_store=pf_store, srcs=srcs, decl_o=decl_o, refs_o=refs_o,
+                                            st_lds=st_lds_k, st_glb=st_glb))
+            return out
         out.append("""
 template <int K>
 __global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict__ st_in, char* __restrict__ st_out%(offp)s) {
@@ -1683,7 +1699,96 @@ __global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict_
                 "    g_rt.launches += 1; steps -= k;\n  }\n}" % (self.k, self.k))
 
 
-def _stream3d_ok(em, ty, tz, cx, k):
+# stream3d with skewed levels (spec key skew): in trip t level j computes plane
+# p = t - (j - 1) from planes p - 2s .. p of level j - 1, all finished in earlier
+# trips; the input plane t + 1 enters its slot at the top of the trip.  Rings of
+# R = 2s + 2 planes: a level's plane is written while the next level reads the
+# 2s + 1 planes before it.  One barrier per trip.
+STREAM3D_SKEW = """
+template <int K>
+__global__ void __launch_bounds__(%(NTH)d) k_step_stream(const char* __restrict__ st_in, char* __restrict__ st_out) {
+  constexpr int TY = %(TY)d, TZ = %(TZ)d, CX = %(CX)d, R = %(R)d;
+  constexpr int H = 2 * K * SLEEVE, RY = TY + H, RZ = TZ + H, PLANE = RY * RZ;
+  constexpr int NTY = N2 / TY, NTZ = N3 / TZ, NCX = N1 / CX, NT = NTY * NTZ * NCX;
+  // levels 0..K-1 live in LDS rings of R planes (level K goes to HBM)
+%(lds)s
+  const int b = blockIdx.x;
+  const int tile = (NT %% 8 == 0) ? ((b %% 8) * (NT / 8) + b / 8) : b;
+  const int cz = tile %% NTZ, cy = (tile / NTZ) %% NTY, cxi = tile / (NTZ * NTY);
+  const int x0 = cxi * CX, y0 = cy * TY, z0 = cz * TZ;
+  const int base = x0 - H;  // first plane of the pipeline: plane p lives in slot (p - base) %% R
+  constexpr int PIT = (PLANE + %(NTH)d - 1) / %(NTH)d;
+  %(pf_decl)s
+  auto prefetch = [&](int t) {
+    const long long gx = (long long)%(w0)s(t) * N2;
+#pragma unroll
+    for (int it = 0; it < PIT; ++it) {
+      const int idx = threadIdx.x + it * %(NTH)d;
+      if (idx < PLANE) {
+        const int ly = idx / RZ, lz = idx - ly * RZ;
+        const long long g = (gx + %(w1)s(y0 - H + ly)) * N3 + %(w2)s(z0 - H + lz);
+%(pf_load)s
+      }
+    }
+  };
+  auto put0 = [&](int slot) {  // the prefetched input plane into its slot
+#pragma unroll
+    for (int it = 0; it < PIT; ++it) {
+      const int idx = threadIdx.x + it * %(NTH)d;
+      if (idx < PLANE) {
+%(pf_store)s
+      }
+    }
+  };
+  prefetch(base);
+  put0(0);
+  prefetch(base + 1);
+  __syncthreads();
+  int rs = 0;  // (t - base) %% R
+  for (int t = base; t < x0 + CX + K - 1; ++t) {
+    if (t + 1 < x0 + CX) {
+      // input plane t + 1 into the slot of plane t + 1 - R, whose last read (by
+      // level 1, in the trip before) ended at the last barrier; level 1 reads
+      // planes t - 2s .. t in this trip.  The next plane's loads are issued
+      // before this trip's work so that they overlap it.
+      put0(rs + 1 == R ? 0 : rs + 1);
+      if (t + 2 < x0 + CX) prefetch(t + 2);
+    }
+#pragma unroll
+    for (int j = 1; j <= K; ++j) {
+      const int p = t - (j - 1);  // the plane level j computes in this trip
+      if (p >= x0 - 2 * (K - j) * SLEEVE && p < x0 + CX) {
+        int pl[2 * SLEEVE + 1];
+#pragma unroll
+        for (int d = 0; d <= 2 * SLEEVE; ++d) {  // slot of plane p - 2s + d
+          const int q = rs + ((d - 2 * SLEEVE - (j - 1)) %% R + R) %% R;
+          pl[d] = (q >= R ? q - R : q) * PLANE;
+        }
+        const int wq = rs + ((-(j - 1)) %% R + R) %% R, ws = wq >= R ? wq - R : wq;  // slot of plane p
+        const int lo = 2 * j * SLEEVE, W = RZ - lo, HH = RY - lo;
+        for (int idx = threadIdx.x; idx < HH * W; idx += %(NTH)d) {
+          const int rr = idx / W;
+          const int ly = lo + rr, lz = lo + (idx - rr * W);
+          const int o = ly * RZ + lz;
+          %(decl_o)s
+          pt_step3<RZ>(%(srcs)s, pl, o, %(refs_o)s);
+          if (j < K) {
+            %(st_lds)s
+          } else {
+            const long long g = ((long long)p * N2 + (y0 + ly - H)) * N3 + (z0 + lz - H);
+            %(st_glb)s
+          }
+        }
+      }
+    }
+    __syncthreads();
+    rs = rs + 1 == R ? 0 : rs + 1;
+  }
+}
+"""
+
+
+def _stream3d_ok(em, ty, tz, cx, k, skew=False):
     plan = em.step
     if em.dim != 3 or plan.temps:
         return False
@@ -1696,8 +1801,10 @@ def _stream3d_ok(em, ty, tz, cx, k):
     s = plan.sleeve
     if 2 * k * s >= min(em.N):
         return False
+    if skew and getattr(em, "tb_index", False):
+        return False
     plane = (ty + 2 * k * s) * (tz + 2 * k * s)
-    lds = sum(8 if ct == "double" else 4 for _, ct in em.fields) * k * (2 * s + 1) * plane
+    lds = sum(8 if ct == "double" else 4 for _, ct in em.fields) * k * (2 * s + (2 if skew else 1)) * plane
     return lds <= LDS_PER_WORKGROUP
 
 
@@ -1769,9 +1876,10 @@ def schedule_from_spec(em, spec: dict):
         return Staged3DSchedule(*args, threads=int(spec.get("threads", 256)))
     if kind == "stream3d":
         args = (int(spec["ty"]), int(spec["tz"]), int(spec["cx"]), int(spec["k"]))
-        if not _stream3d_ok(em, *args):
+        skew = bool(spec.get("skew", 0))
+        if not _stream3d_ok(em, *args, skew=skew):
             raise EmitError("stream3d %r does not apply to %s" % (spec, em.name))
-        return Stream3DSchedule(*args, threads=int(spec.get("threads", 256)))
+        return Stream3DSchedule(*args, threads=int(spec.get("threads", 256)), skew=skew)
     raise EmitError("unknown schedule kind %r" % (kind,))
 
 
@@ -1847,6 +1955,16 @@ def candidate_specs(em) -> List[dict]:
                         out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": k})
                         if ty * tz >= 512 and k in (1, 2, 3) and cx >= 32:
                             out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": k, "threads": 512})
+        # the skewed pipeline (one barrier per plane) on large cross-sections
+        # and long columns, where its extra ring plane still fits
+        for ty, tz in ((32, 64), (64, 32), (16, 64), (32, 32), (16, 32)):
+            for cx in (32, 64, 128, 256):
+                for k in (1, 2, 3):
+                    if not _stream3d_ok(em, ty, tz, cx, k, skew=True):
+                        continue
+                    if (em.N[0] // cx) * (em.N[1] // ty) * (em.N[2] // tz) >= 128:
+                        out.append({"kind": "stream3d", "ty": ty, "tz": tz, "cx": cx, "k": k, "skew": 1,
+                                    "threads": 1024 if ty * tz >= 1024 else 512})
     return out
 
 

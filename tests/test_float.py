@@ -183,7 +183,8 @@ def test_kahan_heat_interpreter_matches_hand_transcription_and_beats_plain_fp32(
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,steps,resident", [("heat3d_kahan_16", 9, False), ("heat3d_kahan_16_stream", 11, True)])
+@pytest.mark.parametrize("name,steps,resident", [("heat3d_kahan_16", 9, False), ("heat3d_kahan_16_stream", 11, True),
+                                                  ("heat3d_kahan_16_skew", 11, True)])
 def test_kahan_heat_library_bit_exact(name, steps, resident):
     print("%s: %d steps" % (name, steps), flush=True)
     mm = kahan_program()

@@ -139,6 +139,9 @@ def test_resident_multi_step_launches(name, steps):
     ("heat3d_32_stream", "heat3d_32", 11, False),     # stream3d, 1 step per launch
     ("heat3d_32_stream", "heat3d_32", 23, True),      # stream3d, 2 steps per launch + remainder
     ("heat3d_64_tb_stream", "heat3d_64_tb", 7, True),  # 4 steps per Forward, 3 per launch
+    ("heat3d_32_skew", "heat3d_32", 11, False),       # skewed stream3d: one barrier per plane
+    ("heat3d_32_skew", "heat3d_32", 23, True),
+    ("heat3d_64_tb_skew", "heat3d_64_tb", 7, True),   # skewed, 3 levels in flight
     ("gs_64_flat", "gs_64", 33, False),
     ("gs_64_tile_k8", "gs_64", 61, True),             # 8 steps per launch, halo 16 of a 64 grid
     ("gs_64_tile_k5_inplace", "gs_64", 47, True),     # one LDS buffer per field, levels overwritten in place
@@ -173,6 +176,7 @@ def test_forced_schedules_bit_exact(name, oracle, steps, resident):
     ("gs_72x120_tile_colmap", "gs_72x120", 35, True),       # 24 x 40 tiles, a fixed column per work-item
     ("heat3d_24x40x56", "heat3d_24x40x56", 13, False),      # default (flat) schedule
     ("heat3d_24x40x56_stream", "heat3d_24x40x56", 15, True),  # 8 x 8 columns along a 24-plane axis
+    ("heat3d_24x40x56_skew", "heat3d_24x40x56", 15, True),    # the skewed pipeline on the same grid
     ("heat3d_24x40x56_tb", "heat3d_24x40x56_tb", 5, False),  # 2 steps per Forward, blocks 7 x 11 x 6
     ("heat3d_24x40x56_tb_stream", "heat3d_24x40x56_tb", 6, True),
 ])
