@@ -150,11 +150,17 @@ class ReverseSweep:
 class AdjointEmitter:
     def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256,
                  tile=(32, 32, 4), adj_tile=(32, 32), levels=4, row_block=1, fwd_threads=None, adj_threads=512,
-                 symbolic: bool = True, fused: bool = True, adj_prefetch: bool = True, adj_index_arrays: bool = True):
+                 symbolic: bool = True, fused: bool = True, adj_prefetch: bool = True, adj_index_arrays: bool = True,
+                 unit_inplace=None):
         self.mm, self.om, self.name = mm, mm.om, name
         self.symbolic = bool(symbolic)
         # units of adjoint steps through k_adjoint_unit (odd levels recomputed in the tile)
         self.fused_ok = bool(fused)
+        # k_adjoint_unit: lam updated in place in one LDS buffer per field (else
+        # ping-pong).  Default: in place for double, whose two lam buffers would
+        # hold a CU to two workgroups (in float four fit either way, and the extra
+        # barrier per level costs more than it gains)
+        self.unit_inplace = (real == "double") if unit_inplace is None else bool(unit_inplace)
         # pointwise adjoint kernel: fetch the next level's state while a level
         # computes (registers), and keep element indices in registers
         self.adj_prefetch, self.adj_index_arrays = bool(adj_prefetch), bool(adj_index_arrays)
@@ -549,7 +555,10 @@ class AdjointEmitter:
         ax, ay = self.adj_tile
         th = self.adj_threads
         fl = "\n".join("  __shared__ real_t F%d[LH * LW];" % i for i in range(nf))
-        ll = "\n".join("  __shared__ real_t L%d[2][LH * LW];" % i for i in range(nf))
+        inpl = self.unit_inplace
+        ll = "\n".join(("  __shared__ real_t L%d[LH * LW];" if inpl else "  __shared__ real_t L%d[2][LH * LW];") % i
+                       for i in range(nf))
+        lbuf = (lambda i: "L%d" % i) if inpl else (lambda i: "L%d[sb]" % i)
         stage = tile_load("FR * LW", ["const int r = idx / LW, c = idx - r * LW;",
                                       "const long long g = (long long)wrapn(x0 - 2 * SLEEVE + r, N1) * N2 + "
                                       "wrapn(y0 - 2 * SLEEVE + c, N2);"],
@@ -559,17 +568,22 @@ class AdjointEmitter:
             TX=ax, TY=ay, ATH=th, fl=fl, ll=ll, stage=stage,
             lload=tile_load("LH * LW", ["const int r = idx / LW, c = idx - r * LW;",
                                         "const long long g = (long long)wrapn(x0 + r, N1) * N2 + wrapn(y0 + c, N2);"],
-                            [("L%d[0][idx]" % i, "LAM%d(lam_next)[g]" % i) for i in range(nf)], "real_t", "ul",
-                            threads=th),
+                            [(("L%d[idx]" if inpl else "L%d[0][idx]") % i, "LAM%d(lam_next)[g]" % i)
+                             for i in range(nf)], "real_t", "ul", threads=th),
+            inplace="true" if inpl else "false",
+            rdecl=(" ".join("real_t res%d[NIT];" % i for i in range(nf)) if inpl else
+                   "const int sb = mm & 1, db = sb ^ 1;"),
+            wback=" ".join("L%d[lo] = res%d[it];" % (i, i) for i in range(nf)) if inpl else "",
             srcs=", ".join("F%d" % i for i in range(nf)),
             xdecl=" ".join("real_t sv%d;" % i for i in range(nf)),
             xstep=", ".join("sv%d" % i for i in range(nf)),
             xframe=" ".join("sv%d = F%d[fo];" % (i, i) for i in range(nf)),
             xglob=" ".join("sv%d = FLD%d(xs)[g];" % (i, i) for i in range(nf)),
-            xargs=", ".join("sv%d" % i for i in range(nf)), largs=", ".join("L%d[sb]" % i for i in range(nf)),
+            xargs=", ".join("sv%d" % i for i in range(nf)), largs=", ".join(lbuf(i) for i in range(nf)),
             adecl=" ".join("real_t a%d;" % i for i in range(nf)) + " " + " ".join("double q%d;" % p for p in range(np_)),
             aargs=", ".join(["a%d" % i for i in range(nf)] + ["q%d" % p for p in range(np_)]),
-            st_lds=" ".join("L%d[db][lo] = a%d;" % (i, i) for i in range(nf)),
+            st_lds=(" ".join("res%d[it] = a%d;" % (i, i) for i in range(nf)) if inpl else
+                    " ".join("L%d[db][lo] = a%d;" % (i, i) for i in range(nf))),
             st_glb=" ".join("LAM%d(lam_prev)[go] = a%d;" % (i, i) for i in range(nf)),
             pdecl=" ".join("double pgs%d = 0;" % p for p in range(np_)),
             pacc=" ".join("pgs%d += q%d;" % (p, p) for p in range(np_)),
@@ -957,9 +971,11 @@ TILE_ADJ_UNIT = r"""
 // [-2s, T + 2ms), the low-side halo the step reads), level m's state is
 // recomputed at each point of the adjoint region from F with the step's exact
 // instruction list, and level m-1's is read from F.  The adjoint body reads the
-// state at the point only.  lam ping-pongs between two LDS buffers (one barrier
-// per level); level m's work-items cover exactly its region [0, TX + 2ms) x
-// [0, TY + 2ms).
+// state at the point only.  lam either ping-pongs between two LDS buffers (one
+// barrier per level) or is updated in place (LAM_INPLACE: a level's results are
+// held in registers until every work-item has read lam_{m+1}, then written
+// back; one LDS buffer, so more workgroups fit on a CU); level m's work-items
+// cover exactly its region [0, TX + 2ms) x [0, TY + 2ms).
 template <int K>
 __global__ void __launch_bounds__(%(ATH)d) k_adjoint_unit(const char* __restrict__ st0, const char* __restrict__ ev,
                                                       const char* __restrict__ lam_next, char* __restrict__ lam_prev,
@@ -969,6 +985,9 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_unit(const char* __restrict
   constexpr int NTY = N2 / TY, NT = (N1 / TX) * NTY;
   static_assert(K <= 4, "levels 0 and 2 are given, 1 and 3 recomputed");
   static_assert((long long)N1 * N2 < (1LL << 31), "element offsets of a field are ints");
+  constexpr bool LAM_INPLACE = %(inplace)s;
+  // most points a work-item holds in one level (the largest region, m = K - 1)
+  constexpr int NIT = ((TX + 2 * (K - 1) * SLEEVE) * (TY + 2 * (K - 1) * SLEEVE) + %(ATH)d - 1) / %(ATH)d;
 %(fl)s
 %(ll)s
   const int tile = xcd_tile(blockIdx.x, NT);
@@ -980,19 +999,20 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_unit(const char* __restrict
   %(pdecl)s
 #pragma unroll
   for (int mm = 0; mm < K; ++mm) {
-    const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} (buffer mm & 1) -> lam_m
-    const int sb = mm & 1, db = sb ^ 1;
+    const int m = K - 1 - mm;  // level: reverse step m, lam_{m+1} -> lam_m (ping-pong: buffer mm & 1)
     const int HH = TX + 2 * m * SLEEVE, W = TY + 2 * m * SLEEVE;
     // state m: recomputed (odd m) from the frame, read from the frame (even m
     // below an odd level), or read from HBM (even m = K-1)
     const bool recompute = (m & 1) != 0, framed = !recompute && m + 1 < K;
     if (recompute) {
-      if (mm > 0) __syncthreads();  // the level above has read the frame
+      // the level above has read the frame (in place: its write-back barrier)
+      if (!LAM_INPLACE && mm > 0) __syncthreads();
       const char* __restrict__ xs = m == 1 ? st0 : ev;  // level m-1
       const int FR = TX + 2 * (m + 1) * SLEEVE;  // frame rows staged: [-2s, T + 2ms) of the tile
 %(stage)s
     }
     __syncthreads();  // lam_{m+1} and the frame are in LDS
+    %(rdecl)s
 #pragma unroll
     for (int it = 0; it < (HH * W + %(ATH)d - 1) / %(ATH)d; ++it) {
       const int idx = threadIdx.x + it * %(ATH)d;
@@ -1018,6 +1038,18 @@ __global__ void __launch_bounds__(%(ATH)d) k_adjoint_unit(const char* __restrict
           %(st_glb)s
         }
         if (r < TX && c < TY) { %(pacc)s }
+      }
+    }
+    if (LAM_INPLACE && m > 0) {
+      __syncthreads();  // every read of lam_{m+1} and of the frame is done
+#pragma unroll
+      for (int it = 0; it < (HH * W + %(ATH)d - 1) / %(ATH)d; ++it) {
+        const int idx = threadIdx.x + it * %(ATH)d;
+        if (idx < HH * W) {
+          const int r = idx / W, c = idx - r * W;
+          const int lo = r * LW + c;
+          %(wback)s
+        }
       }
     }
   }
