@@ -835,7 +835,8 @@ def main():
     # fused steps) over its mean duration
     achieved = r["min_bytes_per_launch"] / (r["mean_launch_us"] * 1e-6) / 1e9
     traffic = _traffic_from_profile(name)
-    issue = _issue_from_profile("k_step_tile<4>")
+    kernel = "k_step_tile<%d>" % round(r["steps_per_launch"])
+    issue = _issue_from_profile(kernel)
     line = {
         "metric": "GLUPS & % HBM roofline, FP32/batched vs plain FP64; adjoint/forward ratio",
         "value": round(r["glups"], 3),
@@ -856,7 +857,7 @@ def main():
                    "bytes_per_lup": 32, "resident": True},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_step_tile<%d>" % round(r["steps_per_launch"]),
+                     "kernel": kernel,
                      "bytes_model": "state read once and written once per launch: 32 B x cells",
                      "effective_gbs": round(r["glups"] * 32.0, 1),
                      "valu_issue_frac": issue},

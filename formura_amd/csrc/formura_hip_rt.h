@@ -71,10 +71,26 @@ __host__ __device__ __forceinline__ double fmr_divc_fast(double a, double b, dou
 // where a*y and a/b lie in different binades), and one Markstein step
 // q1 = q0 - (b*q0 - a)*y gives RN(a/b).  The generator checks the condition in
 // exact rational arithmetic (b = RN(1e-4) with y = 1e4: |b*y - 1| = 0.043 * 2^-53).
+//
+// The remainder is formed at scale 2^120: r = (b 2^120) q0 - a 2^120, then
+// q1 = q0 - r (y 2^-120) (the scaled constants fold at compile time; scaling by
+// a power of two is exact here).  The fused operation sees the same real
+// number q0 - (b q0 - a) y as unscaled, so wherever the unscaled remainder is
+// exact the result is the same double; the scaled remainder is a multiple of
+// 2^(e_b + 120 - 52) * ulp(q0) >= 2^-1020 and below 2^53 of those, so it is
+// exact for every q0 -- subnormal dividends and quotients included, where the
+// unscaled one would underflow.  For a subnormal quotient Q = a/b: q0 is within
+// 3/4 ulp of Q, and q0 - r y = Q + e (Q - q0) with |e| = |b y - 1| <= 2^-54,
+// while Q lies at least 1/(2M) > 2^-54 ulp from every rounding midpoint (b =
+// M 2^k, M odd and above every subnormal's integer significand), so the fused
+// rounding gives RN(Q) as in the normal range.  Valid for |a| <= 2^900 (a 2^120
+// and a y stay finite): fmr_divc1_in_range.
+__host__ __device__ __forceinline__ bool fmr_divc1_in_range(double a) { return fabs(a) <= 0x1p+900; }
+
 __host__ __device__ __forceinline__ double fmr_divc1_fast(double a, double b, double y) {
   const double q = a * y;
-  const double r = fma(b, q, -a);
-  return fma(-r, y, q);
+  const double r = fma(b * 0x1p120, q, -(a * 0x1p120));
+  return fma(-r, y * 0x1p-120, q);
 }
 
 __host__ __device__ __forceinline__ double fmr_divc(double a, double b, double y) {
@@ -83,7 +99,7 @@ __host__ __device__ __forceinline__ double fmr_divc(double a, double b, double y
 }
 
 __host__ __device__ __forceinline__ double fmr_divc1(double a, double b, double y) {
-  if (__builtin_expect(!fmr_divc_in_range(a), 0)) return a / b;
+  if (__builtin_expect(!fmr_divc1_in_range(a), 0)) return a / b;
   return fmr_divc1_fast(a, b, y);
 }
 
@@ -102,11 +118,11 @@ __host__ __device__ __forceinline__ double fmr_divc_redo(double a, double b, dou
   return fmr_divc_fast(a * (tiny ? 0x1p600 : 1.0), b, y) * (tiny ? 0x1p-600 : 1.0);
 }
 
+// (fmr_divc1_fast covers tiny dividends itself: only huge, Inf and NaN ones
+// are redone, with the division)
 __host__ __device__ __forceinline__ double fmr_divc1_redo(double a, double b, double y, double lo) {
-  const double aa = fabs(a);
-  const bool tiny = aa < 0x1p-900 && aa >= lo;
-  if (__builtin_expect(!(tiny || fmr_divc_in_range(a)), 0)) return a / b;
-  return fmr_divc1_fast(a * (tiny ? 0x1p600 : 1.0), b, y) * (tiny ? 0x1p-600 : 1.0);
+  (void)lo;
+  return fmr_divc1(a, b, y);
 }
 
 // Speculative forms for point functions: the fused sequences without the
@@ -123,13 +139,19 @@ __device__ __forceinline__ unsigned long long fmr_divc_out_of_range(double a) {
          __builtin_amdgcn_ballot_w64(aa != 0.0);
 }
 
+// The lanes whose dividend is outside fmr_divc1_in_range: one comparison
+// (false for NaN, so NaN lanes are counted).
+__device__ __forceinline__ unsigned long long fmr_divc1_out_of_range(double a) {
+  return __builtin_amdgcn_ballot_w64(!(fabs(a) <= 0x1p+900));
+}
+
 __device__ __forceinline__ double fmr_divcs(double a, double b, double y, unsigned long long &bad) {
   bad |= fmr_divc_out_of_range(a);
   return fmr_divc_fast(a, b, y);
 }
 
 __device__ __forceinline__ double fmr_divc1s(double a, double b, double y, unsigned long long &bad) {
-  bad |= fmr_divc_out_of_range(a);
+  bad |= fmr_divc1_out_of_range(a);
   return fmr_divc1_fast(a, b, y);
 }
 

@@ -269,8 +269,14 @@ def redo_lower_bound(b: float) -> float:
 def _divfix_lines(ct, name, src, sg, fn, b, y, cast):
     """A division by a constant as the branch-free fused sequence, computed
     again where a lane of the wave holds a dividend outside its range (a
-    wave-uniform branch): tiny dividends through the scaled fused sequence,
-    the rest with the division (fmr_divc_redo)."""
+    wave-uniform branch).  Two corrections (fmr_divc): tiny dividends through
+    the sequence scaled by 2^600, the rest with the division (fmr_divc_redo);
+    one correction (fmr_divc1, remainder at scale 2^120, exact down to the
+    subnormals): only dividends above 2^900, Inf and NaN, with the division."""
+    if fn == "fmr_divc1":
+        return ["%s %s = %s(%sfmr_divc1_fast(%s, %s, %s));" % (ct, name, cast, sg, src, b, y),
+                "if (__builtin_expect(fmr_divc1_out_of_range(%s) != 0ull, 0)) %s = %s(%sfmr_divc1(%s, %s, %s));"
+                % (src, name, cast, sg, src, b, y)]
     lo = show_double(redo_lower_bound(float(b)))
     return ["%s %s = %s(%s%s_fast(%s, %s, %s));" % (ct, name, cast, sg, fn, src, b, y),
             "if (__builtin_expect(fmr_divc_out_of_range(%s) != 0ull, 0)) %s = %s(%s%s_redo(%s, %s, %s, %s));"

@@ -5,6 +5,8 @@
 // Cases: random dividends over the fast range, dividends whose quotient lies
 // next to a rounding midpoint (the hard cases of correct rounding), stencil-like
 // differences, and specials (signed zeros, subnormals, the range limits, Inf, NaN).
+// The one-correction sequence (remainder at scale 2^120) is checked without a
+// range test on tiny and subnormal dividends, near midpoints included.
 #include "formura_hip_rt.h"
 #include <cmath>
 #include <cstdint>
@@ -14,6 +16,13 @@ static uint64_t s = 0x9E3779B97F4A7C15ull;
 static inline uint64_t nx() { uint64_t z = (s += 0x9E3779B97F4A7C15ull); z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull; z = (z ^ (z >> 27)) * 0x94D049BB133111EBull; return z ^ (z >> 31); }
 static inline uint64_t bits(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
 static inline double dbl(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
+// the remainder formed unscaled (it underflows for tiny dividends): the tiny
+// cases below must catch it
+static inline double divc1_unscaled(double a, double b, double y) {
+  const double q = a * y;
+  return fma(-fma(b, q, -a), y, q);
+}
+static long unscaled_bad = 0;
 int main(int argc, char** argv) {
   long n = argc > 1 ? atol(argv[1]) : 10000000;
   if (argc > 2) s = strtoull(argv[2], nullptr, 0);
@@ -78,6 +87,35 @@ int main(int argc, char** argv) {
         ++tot;
         if (bits(fmr_divc_redo(a, b, y, lo)) != bits(want)) { if (bad < 10) printf("MISMATCH (redo) b=%a a=%a got %a want %a\n", b, a, fmr_divc_redo(a, b, y, lo), want); ++bad; }
         if (one && bits(fmr_divc1_redo(a, b, y, lo)) != bits(want)) { if (bad < 10) printf("MISMATCH (redo, one step) b=%a a=%a\n", b, a); ++bad; }
+        // the one-correction sequence forms its remainder at scale 2^120: exact
+        // without a range test for every |a| <= 2^900, tiny and subnormal too
+        if (one && fmr_divc1_in_range(a)) {
+          ++tot;
+          if (bits(fmr_divc1_fast(a, b, y)) != bits(want)) { if (bad < 10) printf("MISMATCH (one step, tiny) b=%a a=%a got %a want %a\n", b, a, fmr_divc1_fast(a, b, y), want); ++bad; }
+        }
+      }
+      // tiny dividends whose quotient lies next to a rounding midpoint, in the
+      // normal range and on the subnormal grid
+      if (one) {
+        for (long i = 0; i < nr; ++i) {
+          uint64_t r = nx();
+          double m;
+          if (i & 1) {  // quotient a subnormal midpoint: (k + 1/2) 2^-1074
+            m = ldexp((double)((r >> 12) & ((1ull << 51) - 1)) + 0.5, -1074);
+          } else {  // a normal quotient's midpoint, 2^-1022 .. 2^-900 / b
+            double q = ldexp(1.0 + (double)(nx() >> 12) * 0x1p-52, -1022 + (int)(r % 100));
+            m = q + ldexp(1.0, ilogb(q) - 53);
+          }
+          double a = b * m;
+          if (r & 4) a = nextafter(a, 0.0);
+          if (r & 8) a = nextafter(a, 1.0);
+          if (r & 16) a = -a;
+          if (!(fabs(a) <= 0x1p-890)) continue;
+          const double want = a / b;
+          ++tot;
+          if (bits(fmr_divc1_fast(a, b, y)) != bits(want)) { if (bad < 10) printf("MISMATCH (one step, tiny midpoint) b=%a a=%a got %a want %a\n", b, a, fmr_divc1_fast(a, b, y), want); ++bad; }
+          unscaled_bad += bits(divc1_unscaled(a, b, y)) != bits(want);
+        }
       }
     }
     double sp[] = {0.0, -0.0, 1e-310, -1e-310, 1e308, -1e308, INFINITY, -INFINITY, NAN, 0x1p-900, 0x1p900, 0x1p-901, 0x1.fffffffffffffp900, 5e-324};
@@ -85,12 +123,16 @@ int main(int argc, char** argv) {
     for (double a : sp) {
       const bool want = a == 0.0 || fabs(a) == 0x1p-900 || fabs(a) == 0x1p900;
       if (fmr_divc_in_range(a) != want) { printf("RANGE MISMATCH a=%a\n", a); ++bad; }
+      // the one-correction range: every |a| <= 2^900 (zeros, subnormals, tiny)
+      const bool want1 = fabs(a) <= 0x1p900;
+      if (fmr_divc1_in_range(a) != want1) { printf("RANGE MISMATCH (one step) a=%a\n", a); ++bad; }
     }
     for (double a : sp) {
       double q1 = fmr_divc(a, b, y), q2 = a / b, q3 = one ? fmr_divc1(a, b, y) : q2;
       ++tot;
       if (bits(q1) != bits(q2) && !(q1 != q1 && q2 != q2)) { printf("SPECIAL MISMATCH b=%a a=%a got %a want %a\n", b, a, q1, q2); ++bad; }
       if (bits(q3) != bits(q2) && !(q3 != q3 && q2 != q2)) { printf("SPECIAL MISMATCH (one step) b=%a a=%a got %a want %a\n", b, a, q3, q2); ++bad; }
+      if (one && fmr_divc1_in_range(a) && bits(fmr_divc1_fast(a, b, y)) != bits(q2)) { printf("SPECIAL MISMATCH (one step, fast) b=%a a=%a\n", b, a); ++bad; }
       int e;
       const double mb = frexp(b, &e);
       const double q5 = fmr_divc_redo(a, b, y, ldexp(1.0, (mb == 0.5 ? e - 1 : e) - 1021));
@@ -99,5 +141,6 @@ int main(int argc, char** argv) {
   }
   printf("divisors %d, one-step divisors %ld\n", NFIX + NRND, one_step);
   printf("checked %ld, mismatches %ld\n", tot, bad);
-  return bad != 0;
+  printf("unscaled remainder, tiny midpoint cases wrong: %ld\n", unscaled_bad);
+  return bad != 0 || unscaled_bad == 0;
 }

@@ -10,7 +10,11 @@ kernels themselves with the oracle.
 
 ``fmr_divc1`` (one correction) is emitted only for divisors with
 |b*RN(1/b) - 1| <= 2^-54, the condition under which q0 = RN(a*y) is faithful
-and Markstein's theorem applies; the checker tests it on those divisors.  A
+and Markstein's theorem applies; the checker tests it on those divisors.  Its
+remainder is formed at scale 2^120, which keeps it exact down to subnormal
+dividends and quotients: the checker runs it without a range test on tiny and
+subnormal dividends (near rounding midpoints too) for every |a| <= 2^900, and
+requires the unscaled remainder to fail some of those cases.  A
 failure of one correction elsewhere needs a quotient within about 2^-52 ulp of
 a rounding midpoint, which random and near-midpoint dividends do not reach: the
 condition rests on the proof in the header, not on this test.
@@ -53,10 +57,12 @@ def test_emitted_divisions():
     from formura_amd.emit_hip import generate
     _, src, _ = generate(os.path.join(build.PROG_DIR, "grayscott2d.fmr"),
                          os.path.join(build.PROG_DIR, "gs_64.yaml"), "gs_64")
-    # d_xx and d_yy of u and of v: the fused sequence, and the guarded form
-    # for the waves that hold a dividend outside its range
-    assert src.count("fmr_divc1_fast(") == 4 and src.count("fmr_divc1_redo(") == 4
-    assert src.count(", 0.0001, 10000.0") == 8 and src.count("fmr_divc_out_of_range(") == 4
+    # d_xx and d_yy of u and of v: the fused sequence (remainder at scale
+    # 2^120), and the guarded form for the waves that hold a dividend above
+    # 2^900 (one comparison per division)
+    assert src.count("fmr_divc1_fast(") == 4 and src.count("fmr_divc1(") == 4
+    assert src.count(", 0.0001, 10000.0") == 8 and src.count("fmr_divc1_out_of_range(") == 4
+    assert "fmr_divc_out_of_range(" not in src and "fmr_divc1_redo(" not in src
     _, fsrc, _ = generate(os.path.join(build.PROG_DIR, "grayscott2d.fmr"),
                           os.path.join(build.PROG_DIR, "gs_64.yaml"), "gs_64", "flat")
     assert fsrc.count("fmr_divc1(") == 4 and "fmr_divc1s(" not in fsrc
