@@ -33,7 +33,7 @@ import math
 from typing import Dict, List
 
 from . import syntax as S
-from .emit import Body, EmitError, tile_load, GraphPlan, _const_literal, _f32, show_double, state_fields
+from .emit import Body, EmitError, LDS_LOAD_LINE, tile_load, GraphPlan, _const_literal, _f32, show_double, state_fields
 from .manifest import calc_range, inline_temporaries
 from .symadj import SymbolicAdjoint, Unsupported
 
@@ -237,8 +237,12 @@ class AdjointEmitter:
             return xload(self.fi[field], [shift[a] + c[a] + base for a in range(self.dim)])
 
         pmap = {p: "P[%d]" % i for i, p in enumerate(self.params)}
+        # exact FP64 bodies divide by constants without a branch (the fused
+        # sequence; a wave holding a dividend out of its range redoes that
+        # division, a wave-uniform branch), as the plain path's tile kernels do
         return Body(self.plan.mm[k], load, lambda *a: None, lambda ax: None, lambda n, v: "", prefix,
-                    params=pmap, real=("float" if self.real == "float" else None), recip=not exact)
+                    params=pmap, real=("float" if self.real == "float" else None), recip=not exact,
+                    divfix=exact and self.real == "double")
 
     def deltas(self):
         """Distinct load offsets delta = c + base over all stores and fields."""
@@ -496,15 +500,18 @@ class AdjointEmitter:
         tx, ty, kmax = tile
         ax, ay = self.adj_tile
         # forward point function on LDS levels
-        pbody = []
+        pbody, ploads = [], []
         for si, k in enumerate(self.plan.stores):
             node = self.plan.mm[k]
             fb = self._forward_body(k, [0, 0], "s%d_" % si,
                                     lambda f, a: "lds%d[o + (%d)*RY + (%d)]" % (f, a[0], a[1]))
+            # the node's LDS loads (names unique to the node) before every node's arithmetic
+            ploads += [l for l in fb.lines if LDS_LOAD_LINE.search(l)]
             pbody.append("{")
-            pbody += ["  " + l for l in fb.lines]
+            pbody += ["  " + l for l in fb.lines if not LDS_LOAD_LINE.search(l)]
             pbody.append("  o%d = s%d_%d;" % (self.fi[node.micro[-1].inst[1]], si, node.micro[-1].inst[2]))
             pbody.append("}")
+        pbody = ploads + pbody
         lparams = ", ".join("const real_t* __restrict__ lds%d" % i for i in range(nf))
         outs = ", ".join("real_t& o%d" % i for i in range(nf))
         src = []
