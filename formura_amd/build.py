@@ -119,8 +119,9 @@ PROGRAMS: Dict[str, tuple] = {
     "gs_72x120_tile_colmap": ("grayscott2d.fmr", "gs_72x120.yaml",
                               {"kind": "tile2d", "tx": 24, "ty": 40, "k": 3, "colmap": 1}),
     # the opt-in contract form (FMA contraction, reciprocal division; not bit-exact)
-    "gs_2048_contract": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "tile2d", "tx": 32, "ty": 32, "k": 4,
-                                                              "threads": 512, "contract": 1}),
+    "gs_2048_contract": ("grayscott2d.fmr", "gs_2048.yaml", {"kind": "tile2d", "tx": 32, "ty": 64, "k": 5,
+                                                              "threads": 1024, "inplace": 1, "colmap": 1,
+                                                              "contract": 1}),
     "euler3d_256_contract": ("euler3d.fmr", "euler3d_256.yaml", {"kind": "staged3d", "ty": 4, "tz": 64, "cx": 32,
                                                                   "threads": 512, "inline": "inner", "contract": 1}),
     "gs_64_contract": ("grayscott2d.fmr", "gs_64.yaml", {"kind": "tile2d", "tx": 16, "ty": 32, "k": 4, "contract": 1}),
