@@ -151,7 +151,7 @@ class AdjointEmitter:
     def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256,
                  tile=(32, 32, 4), adj_tile=(32, 32), levels=4, row_block=1, fwd_threads=None, adj_threads=512,
                  symbolic: bool = True, fused: bool = True, adj_prefetch: bool = True, adj_index_arrays: bool = True,
-                 unit_inplace=None):
+                 unit_inplace=None, unit_next: bool = True):
         self.mm, self.om, self.name = mm, mm.om, name
         self.symbolic = bool(symbolic)
         # units of adjoint steps through k_adjoint_unit (odd levels recomputed in the tile)
@@ -161,6 +161,9 @@ class AdjointEmitter:
         # hold a CU to two workgroups (in float four fit either way, and the extra
         # barrier per level costs more than it gains)
         self.unit_inplace = (real == "double") if unit_inplace is None else bool(unit_inplace)
+        # k_adjoint_unit_next: a unit's adjoint and, in the same launch, the
+        # level-2 state of the unit reversed next (when the executor has it)
+        self.unit_next = bool(unit_next)
         # pointwise adjoint kernel: fetch the next level's state while a level
         # computes (registers), and keep element indices in registers
         self.adj_prefetch, self.adj_index_arrays = bool(adj_prefetch), bool(adj_index_arrays)
@@ -527,8 +530,9 @@ class AdjointEmitter:
         st_lds = " ".join("B%d[db][o] = o%d;" % (i, i) for i in range(nf))
         st_lvl = " ".join("FLD%d(lvl)[g] = o%d;" % (i, i) for i in range(nf))
         st_glb = " ".join("FLD%d(dst)[g] = o%d;" % (i, i) for i in range(nf))
-        src.append(TILE_FWD % dict(TX=tx, TY=ty, FTH=self.fwd_threads, lds=lds, load_tile=load_tile, srcs=srcs, decl=decl, refs=refs,
-                                   st_lds=st_lds, st_lvl=st_lvl, st_glb=st_glb))
+        fwd_args = dict(TX=tx, TY=ty, lds=lds, srcs=srcs, decl=decl, refs=refs, st_lds=st_lds, st_lvl=st_lvl,
+                        st_glb=st_glb)
+        src.append(TILE_FWD % dict(fwd_args, FTH=self.fwd_threads, load_tile=load_tile))
         if self.pointwise_state():
             src.append(self._emit_adj_pointwise(uses_state))
         else:
@@ -547,8 +551,62 @@ class AdjointEmitter:
                          "lam_prev, P, pgrad); break;\n" % (k, k, self.adj_threads) for k in range(1, self.levels + 1))
         src.append(TILE_HOST % dict(TX=tx, TY=ty, AX=ax, AY=ay, K=kmax, fcases=fcases, lcases=lcases, acases=acases))
         if ucases:
-            src.append(UNIT_HOST % dict(ucases=ucases))
+            src.append(FWD_JOB)
+            if self.unit_next:
+                src.append(self._emit_unit_next(uses_state, fwd_args))
+            src.append(UNIT_HOST % dict(ucases=ucases, ncases=self._unit_next_cases(),
+                                        has_next="true" if self.unit_next else "false"))
         return "\n".join(src)
+
+    def _unit_next_cases(self):
+        if not self.unit_next:
+            return ""
+        return "".join("    case %d: hipLaunchKernelGGL(k_adjoint_unit_fwd<%d>, dim3(NTA + nj * NT), dim3(%d), 0, s, st0, ev, "
+                       "lam_next, lam_prev, P, pgrad, nj, j0, j1); break;\n" % (k, k, self.adj_threads)
+                       for k in range(1, self.levels + 1))
+
+    def _emit_unit_next(self, uses_state, fwd_args):
+        """k_adjoint_unit_fwd<K>: the workgroups [0, NTA) run k_adjoint_unit<K>'s
+        tiles, the workgroups after them the forward tiles of up to two forward
+        jobs (2 or 4 steps: a level-2 state, a recomputation launch) that the
+        executor found independent of the adjoint.  The block dispatcher starts
+        the forward tiles as the adjoint's last round leaves slots free.  Both
+        parts are the kernels' own bodies as device functions over one LDS pool
+        (the larger of the two footprints)."""
+        nf = len(self.fields)
+        s = self.plan.sleeve
+        tx, ty, _ = self.tile
+        ax, ay = self.adj_tile
+        th = self.adj_threads
+        if (tx, ty) != (ax, ay) or self.tile[2] < 4:
+            raise EmitError("k_adjoint_unit_fwd needs the same forward and adjoint tile and 4-step forward launches")
+        lt = tile_load("RX * RY", ["const int r = idx / RY, c = idx - r * RY;",
+                                   "const long long g = (long long)wrapn(x0 - H + r, N1) * N2 + wrapn(y0 - H + c, N2);"],
+                       [("B%d[0][idx]" % i, "FLD%d(st_in)[g]" % i) for i in range(nf)], "real_t", "ld", threads=th)
+        fwd = TILE_FWD % dict(fwd_args, FTH=th, load_tile=lt)
+        fwd = fwd[fwd.index("template <int K, bool ALL>"):]
+        fwd = _device_part(fwd, "k_forward_tile", "fwd_part",
+                           ["  __shared__ real_t B%d[2][RX * RY];" % i for i in range(nf)],
+                           ["  real_t (*B%d)[RX * RY] = reinterpret_cast<real_t (*)[RX * RY]>(pool + %d * RX * RY);"
+                            % (i, 2 * i) for i in range(nf)])
+        adj = self._emit_adj_unit(uses_state)
+        nl = 1 if self.unit_inplace else 2
+        lpool = ((["  real_t* L%d = pool + %d * LH * LW;" % (i, nf + i) for i in range(nf)]) if self.unit_inplace else
+                 (["  real_t (*L%d)[LH * LW] = reinterpret_cast<real_t (*)[LH * LW]>(pool + %d * LH * LW);"
+                   % (i, nf + 2 * i) for i in range(nf)]))
+        adj = _device_part(adj, "k_adjoint_unit", "adj_part",
+                           ["  __shared__ real_t F%d[LH * LW];" % i for i in range(nf)]
+                           + [("  __shared__ real_t L%d[LH * LW];" if self.unit_inplace else
+                               "  __shared__ real_t L%d[2][LH * LW];") % i for i in range(nf)],
+                           ["  real_t* F%d = pool + %d * LH * LW;" % (i, i) for i in range(nf)] + lpool)
+        lh, lw = ax + 2 * self.levels * s, ay + 2 * self.levels * s
+        pool = max(nf * (1 + nl) * lh * lw, nf * 2 * (tx + 8 * s) * (ty + 8 * s))
+        # as many workgroups per CU as the unit kernel's LDS admits (the register
+        # budget of the two parts together would otherwise allow fewer)
+        # (HIP's second launch bound is the minimum number of waves per SIMD)
+        esize = 8 if self.real == "double" else 4
+        minb = max(1, min(2048 // th, (160 * 1024) // (pool * esize + 1024)))
+        return "\n".join([fwd, adj, UNIT_NEXT % dict(ATH=th, POOL=pool, MINW=max(1, minb * th // 256))])
 
     def _emit_adj_unit(self, uses_state):
         """k_adjoint_unit<K>: K (<= 4) adjoint levels over the adjoint tile; the
@@ -682,6 +740,63 @@ class AdjointEmitter:
             pgs=", ".join("pgs%d" % p for p in range(np_))))
         return "\n".join(src)
 
+
+def _device_part(text: str, kernel: str, name: str, lds_lines: List[str], pool_lines: List[str]) -> str:
+    """A generated kernel as a device function ``name`` over an LDS pool: its
+    signature gains ``const int bid`` (the workgroup's tile number) and
+    ``real_t* pool``; its LDS arrays become pointers into the pool."""
+    head = text.index("__global__ void __launch_bounds__(")
+    open_brace = text.index(") {", head)
+    params = text[text.index(kernel + "(", head) + len(kernel) + 1:open_brace]
+    text = (text[:head] + "__device__ __forceinline__ void %s(const int bid, real_t* __restrict__ pool, %s) {"
+            % (name, " ".join(params.split())) + text[open_brace + 3:])
+    for old, new in zip(lds_lines, pool_lines):
+        assert text.count(old + "\n") == 1, old
+        text = text.replace(old + "\n", new + "\n")
+    text = text.replace("xcd_tile(blockIdx.x, NT)", "xcd_tile(bid, NT)")
+    text = text.replace("(long long)blockIdx.x * NP", "(long long)bid * NP")
+    if "blockIdx" in text or "__shared__ real_t" in text:
+        raise EmitError("%s: workgroup index or LDS array left in the device part" % name)
+    return text
+
+
+FWD_JOB = r"""
+// A forward job riding in an adjoint launch: k (2 or 4) steps from `in` into `out`.
+struct FwdJob {
+  const char* in;
+  char* out;
+  int k;
+};
+"""
+
+UNIT_NEXT = r"""
+// The adjoint of K steps (workgroups [0, NTA), k_adjoint_unit's tiles) and nj
+// (0..2) forward jobs (NT workgroups each, the forward tiles) in one launch: the
+// forward tiles take the slots the adjoint's last round of workgroups leaves
+// free.  The executor only pairs work that touches disjoint buffers.
+template <int K>
+__global__ void __launch_bounds__(%(ATH)d, %(MINW)d) k_adjoint_unit_fwd(const char* __restrict__ st0, const char* __restrict__ ev,
+                                                          const char* __restrict__ lam_next, char* __restrict__ lam_prev,
+                                                          const double* __restrict__ P, double* __restrict__ pgrad,
+                                                          const int nj, const FwdJob j0, const FwdJob j1) {
+  __shared__ real_t pool[%(POOL)d];
+  // role 0: adjoint tile t; role q + 1: tile t of job q.  Every adjoint tile
+  // comes first, so the forward tiles fill the adjoint's last round (measured:
+  // interleaving the roles within each XCD made the reversal 13 %% slower)
+  const int b = (int)blockIdx.x;
+  const int role = b < NTA ? 0 : 1 + (b - NTA) / NT, t = b < NTA ? b : (b - NTA) %% NT;
+  if (role > nj) return;
+  if (role == 0) {
+    adj_part<K>(t, pool, st0, ev, lam_next, lam_prev, P, pgrad);
+  } else {
+    const FwdJob& J = role == 1 ? j0 : j1;
+    if (J.k == 2)
+      fwd_part<2, false>(t, pool, J.in, J.out, P);
+    else
+      fwd_part<4, false>(t, pool, J.in, J.out, P);
+  }
+}
+"""
 
 TILE_FWD = r"""
 // XCD-aware tile order: workgroup ids go round-robin to the 8 XCDs, so each XCD
@@ -1070,6 +1185,16 @@ static void fwd_even(const char*, char*, const double*, hipStream_t) { fmr_rt::f
 static void adj_unit(int, const char*, const char*, const char*, char*, const double*, double*, hipStream_t) {
   fmr_rt::fatal("adj_unit: no fused unit");
 }
+struct FwdJob {
+  const char* in;
+  char* out;
+  int k;
+};
+constexpr bool HAS_UNIT_FWD = false;
+static void adj_unit_fwd(int, const char*, const char*, const char*, char*, const double*, double*, int, FwdJob, FwdJob,
+                         hipStream_t) {
+  fmr_rt::fatal("adj_unit_fwd: no fused unit");
+}
 """
 
 UNIT_HOST = r"""
@@ -1083,6 +1208,20 @@ static void adj_unit(int k, const char* st0, const char* ev, const char* lam_nex
                      double* pgrad, hipStream_t s) {
   switch (k) {
 %(ucases)s    default: fmr_rt::fatal("bad adjoint step count");
+  }
+}
+
+// adj_unit and, in the same launch, nj (0..2) forward jobs (k_adjoint_unit_fwd)
+constexpr bool HAS_UNIT_FWD = %(has_next)s;
+static void adj_unit_fwd(int k, const char* st0, const char* ev, const char* lam_next, char* lam_prev,
+                         const double* P, double* pgrad, int nj, FwdJob j0, FwdJob j1, hipStream_t s) {
+  if (nj < 0 || nj > 2) fmr_rt::fatal("adj_unit_fwd: bad job count");
+  for (int q = 0; q < nj; ++q) {
+    const FwdJob& J = q == 0 ? j0 : j1;
+    if (J.k != 2 && J.k != 4) fmr_rt::fatal("adj_unit_fwd: a forward job is 2 or 4 steps");
+  }
+  switch (k) {
+%(ncases)s    default: fmr_rt::fatal("bad adjoint step count");
   }
 }
 """
@@ -1264,6 +1403,100 @@ extern "C" void Formura_AD_Adjoint_Levels(const void* state0, const void* scratc
   FMR_CHECK(hipGetLastError());
 }
 
+// The reversal on one stream with a fused unit: the operations in program order,
+// each unit's adjoint launch also carrying up to two forward jobs of later
+// operations -- a unit's level-2 state, or a 2- or 4-step recomputation --
+// that are ready and touch nothing pending.  A job at position j may run now if
+// every operation before j that touches its output buffer, and every one that
+// writes its input buffer, was issued in an earlier launch (so, in stream
+// order, has finished); the adjoint it rides with is such an operation, so it
+// never shares a buffer with the job.  Buffers: the states 0..nbufs-1 and the
+// two scratch halves nbufs, nbufs+1 (a unit's level-2 state).  Returns as
+// Formura_AD_Reverse.
+static int reverse_fused(const int* ops, int nops, void* const* bufs, int nbufs, void* tmp, char* const* sc,
+                         char* lam, char* lam2, const double* P, double* pgrad, hipStream_t s) {
+  const int nb = nbufs + 2;
+  std::vector<int> half(nops, 0);
+  std::vector<char> adj_done(nops, 0), lvl_done(nops, 0);  // adjoint issued; level-2 state issued (or none needed)
+  std::vector<std::vector<int>> touch(nb), writer(nb);
+  int turn = 0;
+  for (int i = 0; i < nops; ++i) {
+    const int kind = ops[4 * i], a = ops[4 * i + 1], b = ops[4 * i + 2], k = ops[4 * i + 3];
+    if (a < 0 || a >= nbufs || (kind == 0 && (b < 0 || b >= nbufs || b == a)) || k < 1 || (kind == 1 && k > KA))
+      fmr_rt::fatal("Formura_AD_Reverse: bad operation");
+    if (kind == 0) {
+      touch[a].push_back(i);
+      touch[b].push_back(i);
+      writer[b].push_back(i);
+      adj_done[i] = lvl_done[i] = 0;
+    } else {
+      half[i] = KA > 1 ? (turn++ & 1) : 0;
+      touch[a].push_back(i);
+      touch[nbufs + half[i]].push_back(i);
+      lvl_done[i] = k <= 2;  // no level-2 state for a unit of one or two steps
+    }
+  }
+  // an operation is issued when all its launches are: a forward one when its
+  // launch is (adj_done doubles as its flag), a unit when its adjoint and its
+  // level-2 state are
+  auto issued = [&](int q) { return ops[4 * q] == 0 ? adj_done[q] != 0 : (adj_done[q] && lvl_done[q]); };
+  std::vector<size_t> tcur(nb, 0), wcur(nb, 0);
+  auto first_open = [&](std::vector<int>& list, size_t& cur) {
+    while (cur < list.size() && issued(list[cur])) ++cur;
+    return cur < list.size() ? list[cur] : nops;
+  };
+  auto ready = [&](int j, int in, int out) {
+    // the cursors skip issued operations; an unissued one before j blocks
+    return first_open(touch[out], tcur[out]) >= j && first_open(writer[in], wcur[in]) >= j;
+  };
+  char* lcur = lam;
+  char* lnxt = lam2;
+  const int WINDOW = 256;
+  for (int i = 0; i < nops; ++i) {
+    const int kind = ops[4 * i], a = ops[4 * i + 1], b = ops[4 * i + 2], k = ops[4 * i + 3];
+    if (kind == 0) {
+      if (!adj_done[i]) {
+        Formura_AD_Forward(bufs[a], bufs[b], tmp, k, P, (void*)s);
+        adj_done[i] = 1;
+      }
+      continue;
+    }
+    if (!lvl_done[i]) {
+      fwd_even((const char*)bufs[a], sc[half[i]], P, s);
+      lvl_done[i] = 1;
+    }
+    FwdJob jb[2] = {{nullptr, nullptr, 0}, {nullptr, nullptr, 0}};
+    int pick[2] = {-1, -1}, nj = 0;
+    for (int j = i + 1; HAS_UNIT_FWD && j < nops && j <= i + WINDOW && nj < 2; ++j) {
+      const int kj = ops[4 * j + 3], aj = ops[4 * j + 1];
+      if (ops[4 * j] == 1) {
+        if (!lvl_done[j] && ready(j, aj, nbufs + half[j])) {
+          jb[nj] = FwdJob{(const char*)bufs[aj], sc[half[j]], 2};
+          pick[nj++] = j;
+        }
+      } else if (!adj_done[j] && (kj == 2 || kj == 4) && ready(j, aj, ops[4 * j + 2])) {
+        jb[nj] = FwdJob{(const char*)bufs[aj], (char*)bufs[ops[4 * j + 2]], kj};
+        pick[nj++] = j;
+      }
+    }
+    if (nj > 0)
+      adj_unit_fwd(k, (const char*)bufs[a], sc[half[i]], lcur, lnxt, P, pgrad, nj, jb[0], jb[1], s);
+    else
+      adj_unit(k, (const char*)bufs[a], sc[half[i]], lcur, lnxt, P, pgrad, s);
+    adj_done[i] = 1;
+    for (int q = 0; q < nj; ++q) {
+      if (ops[4 * pick[q]] == 1)
+        lvl_done[pick[q]] = 1;
+      else
+        adj_done[pick[q]] = 1;
+    }
+    char* t = lcur;
+    lcur = lnxt;
+    lnxt = t;
+  }
+  return lcur == lam ? 0 : 1;
+}
+
 extern "C" int Formura_AD_Reverse(const int* ops, int nops, void* const* bufs, int nbufs, void* tmp, void* scratch,
                                   void* lam, void* lam2, const double* params_dev, double* pgrad, void* stream_a,
                                   void* stream_f) {
@@ -1282,6 +1515,11 @@ extern "C" int Formura_AD_Reverse(const int* ops, int nops, void* const* bufs, i
     FMR_CHECK(hipEventRecord(ev.e[nbufs + 4], A));
     FMR_CHECK(hipStreamWaitEvent(F, ev.e[nbufs + 4], 0));
   }
+  if (FUSED_UNIT && !two) {
+    const int r = reverse_fused(ops, nops, bufs, nbufs, tmp, sc, (char*)lam, (char*)lam2, params_dev, pgrad, A);
+    FMR_CHECK(hipGetLastError());
+    return r;
+  }
   for (int i = 0; i < nops; ++i) {
     const int kind = ops[4 * i], a = ops[4 * i + 1], b = ops[4 * i + 2], k = ops[4 * i + 3];
     if (a < 0 || a >= nbufs || (kind == 0 && (b < 0 || b >= nbufs || b == a)) || k < 1)
@@ -1292,7 +1530,9 @@ extern "C" int Formura_AD_Reverse(const int* ops, int nops, void* const* bufs, i
     } else {  // the adjoint of the k steps from bufs[a], on A (inner states on F, or in the fused kernel)
       if (k > KA) fmr_rt::fatal("Formura_AD_Reverse: unit longer than Formura_AD_Levels()");
       const int h = KA > 1 ? (turn++ & 1) : 0;
-      if (FUSED_UNIT) {  // the level-2 state on F, the unit's adjoint (odd levels recomputed) on A
+      if (FUSED_UNIT && !two) {
+        fmr_rt::fatal("Formura_AD_Reverse: one-stream fused reversal runs in reverse_fused");
+      } else if (FUSED_UNIT) {  // the level-2 state on F, the unit's adjoint (odd levels recomputed) on A
         if (two && half_used[h]) FMR_CHECK(hipStreamWaitEvent(F, ev.e[nbufs + h], 0));
         if (k > 2) fwd_even((const char*)bufs[a], sc[h], params_dev, F);
         if (two) {

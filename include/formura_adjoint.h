@@ -60,7 +60,10 @@ void Formura_AD_Adjoint_Levels(const void *state0, const void *scratch, const vo
  *           lam to lam2 (then the two swap)                      (stream_a)
  * Events order the two streams on every shared buffer, so the forward work of
  * later operations overlaps the adjoint ones (stream_f == stream_a: one
- * stream, no events).  lam holds dL/d(output) on entry; returns 0 if the
+ * stream, no events; a library with the fused adjoint unit then runs up to two
+ * forward launches of later operations -- a unit's level-2 state, a 2- or
+ * 4-step recomputation -- inside each adjoint launch, when every earlier
+ * operation touching their buffers has been issued before it).  lam holds dL/d(output) on entry; returns 0 if the
  * result dL/d(bufs of the first adjoint's state...) ends in lam, 1 if in lam2.
  * pgrad accumulates as in Formura_AD_Adjoint.  Returns after stream_a has
  * finished when the two streams differ. */
