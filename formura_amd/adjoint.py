@@ -155,8 +155,11 @@ def reversal_ops(nsteps: int, unit: int, snapshots: int):
     buffers: 0 holds the initial state, 1..len(first_descent) the states the
     forward sweep kept, the others are work buffers.  (0, a, b, k): buffer b =
     k forward steps from buffer a; (1, a, -1, k): the adjoint of the k steps
-    that start at buffer a.  Returns (ops, number of buffers).  Work buffers are
-    reused oldest-freed first, so forward work can run ahead of the adjoint."""
+    that start at buffer a.  Returns (ops, number of buffers).  A checkpoint's
+    buffer (a kept snapshot's too) is freed once the units after it are
+    reversed, and freed buffers are reused oldest-freed first, so forward work
+    can run ahead of the adjoint; buffer 0 (the caller's initial state) is never
+    written."""
     import collections
     n = int(nsteps)
     nunits = -(-n // unit) if n > 0 else 0
@@ -164,8 +167,10 @@ def reversal_ops(nsteps: int, unit: int, snapshots: int):
     kept_ids = list(range(1, 1 + len(path)))
     ops = []
     first_work = 1 + len(kept_ids)
-    free = collections.deque([first_work, first_work + 1])
-    count = [first_work + 2]
+    # three work buffers to start with: with the buffers freed one run before,
+    # a recomputation chain can run one launch ahead of the adjoints it feeds
+    free = collections.deque([first_work, first_work + 1, first_work + 2])
+    count = [first_work + 3]
 
     def alloc():
         if free:
@@ -196,12 +201,10 @@ def reversal_ops(nsteps: int, unit: int, snapshots: int):
                         adjoint(base, pos)
                 return
             j = split(l, s)
-            owned = not kept
             mid = kept.pop(0) if kept else advance(base, pos, j)
             reverse(mid, pos + j, l - j, s - 1, kept)
             kept = []
-            if owned:
-                free.append(mid)
+            free.append(mid)  # no later operation reads this checkpoint
             l = j
 
     if nunits > 0:

@@ -104,3 +104,26 @@ def test_units_lower_the_repetition_number():
     assert A.repetitions(500, 32) == 2
     assert A.recompute_steps(2000, 32, 4) == 4 * A.forward_steps_needed(500, 32)
     assert A.recompute_steps(2000, 32, 4) < 0.6 * A.recompute_steps(2000, 32, 1)
+
+
+def test_reversal_buffers_and_initial_state():
+    """The adjoint configuration (2000 steps, units of 4, 32 snapshots) holds at
+    most 34 states at once and uses 36 buffers (checkpoint buffers, kept
+    snapshots included, are recycled once the units after them are reversed;
+    two spare work buffers let recomputation run ahead); no operation writes
+    buffer 0, the caller's initial state."""
+    ops, nbufs = A.reversal_ops(2000, 4, 32)
+    assert nbufs <= 36
+    assert all(not (kind == 0 and b == 0) for kind, a, b, k in ops)
+    # every buffer an operation reads holds the state of the position it expects
+    pos = {0: 0}
+    first = A.first_descent(500, 32)
+    p = 0
+    for i, j in enumerate(first):
+        p += j
+        pos[1 + i] = p
+    for kind, a, b, k in ops:
+        assert a in pos, "a read of a buffer that holds no state"
+        if kind == 0:
+            pos[b] = pos[a] + k // 4
+    assert len([o for o in ops if o[0] == 1]) == 500
