@@ -149,7 +149,7 @@ class ReverseSweep:
 
 class AdjointEmitter:
     def __init__(self, mm, name: str, params: List[str], real: str = "double", block: int = 256,
-                 tile=(32, 32, 4), adj_tile=(32, 32), levels=4, row_block=1, fwd_threads=None, adj_threads=512,
+                 tile=(32, 32, 4), adj_tile=(32, 32), levels=4, row_block=1, fwd_threads=None, adj_threads=None,
                  symbolic: bool = True, fused: bool = True, adj_prefetch: bool = True, adj_index_arrays: bool = True,
                  unit_inplace=None, unit_next: bool = True):
         self.mm, self.om, self.name = mm, mm.om, name
@@ -176,10 +176,13 @@ class AdjointEmitter:
         # points per work-item along the first axis in the adjoint levels kernel
         self.row_block = int(row_block)
         # work-items per workgroup of the tiled forward and adjoint kernels (the
-        # forward: 1024 for FP64 state, 512 for FP32, the faster of the two on a
-        # 1024^2 Gray-Scott for each)
+        # forward: 1024 for FP64 state, 512 for FP32; the adjoint unit with the
+        # forward jobs it carries: 512 for FP64, 256 for FP32 -- the faster on a
+        # 1024^2 Gray-Scott reversal for each)
         if fwd_threads is None:
             fwd_threads = 1024 if real == "double" else 512
+        if adj_threads is None:
+            adj_threads = 512 if real == "double" else 256
         self.fwd_threads, self.adj_threads = int(fwd_threads), int(adj_threads)
         self.dim = self.om.dimension
         self.N = self.om.config.grid_per_node
