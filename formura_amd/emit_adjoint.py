@@ -609,7 +609,8 @@ class AdjointEmitter:
         # (HIP's second launch bound is the minimum number of waves per SIMD)
         esize = 8 if self.real == "double" else 4
         minb = max(1, min(2048 // th, (160 * 1024) // (pool * esize + 1024)))
-        return "\n".join([fwd, adj, UNIT_NEXT % dict(ATH=th, POOL=pool, MINW=max(1, minb * th // 256))])
+        # waves per SIMD for minb workgroups of th / 64 waves on 4 SIMDs, rounded up
+        return "\n".join([fwd, adj, UNIT_NEXT % dict(ATH=th, POOL=pool, MINW=max(1, -(-minb * th // 256)))])
 
     def _emit_adj_unit(self, uses_state):
         """k_adjoint_unit<K>: K (<= 4) adjoint levels over the adjoint tile; the
